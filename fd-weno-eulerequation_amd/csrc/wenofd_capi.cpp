@@ -1,0 +1,772 @@
+// wenofd_capi.cpp — the C ABI (include/wenofd_amd.h): solver handles, device
+// buffers, the RK stage sequence and the run loop, outputs.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "../../include/wenofd_amd.h"
+#include "wenofd_host.hpp"
+#include "wenofd_kernels.hpp"
+
+using namespace wenofd;
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(WENOFD_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+namespace {
+
+constexpr int kDtBlocks = 1024;
+
+struct TimedLaunch {
+    int kind;
+    hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct wenofd_solver {
+    Options opt;
+    CaseDef cs;
+    Geom g{};
+    Bc bc{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double* U = nullptr;        // 4 state planes incl. ghosts
+    double* Un = nullptr;       // RK base state
+    double* Fx = nullptr;       // x-face fluxes
+    double* Fy = nullptr;       // y-face fluxes
+    double* rhs = nullptr;      // rhs in the state layout (allocated on first get_flux)
+    double* special = nullptr;  // SPECIAL boundary values [4][2 nx + 2 ny]
+    double* dt_part = nullptr;
+    int* dt_part_nan = nullptr;
+    double* dt_dev = nullptr;   // {min, nan flag}
+    double* dt_host = nullptr;  // pinned copy of dt_dev
+    std::vector<double> xc, yc;
+    double now = 0;
+    long long steps = 0;
+    std::string out_dir;
+    bool timing = false;
+    std::vector<TimedLaunch> events;
+    std::vector<hipEvent_t> event_pool;
+    double t_ms[WENOFD_K_COUNT] = {};
+    long long t_cnt[WENOFD_K_COUNT] = {};
+
+    size_t state_doubles() const { return 4 * (size_t)g.plane; }
+};
+
+namespace {
+
+hipEvent_t take_event(wenofd_solver* h) {
+    if (!h->event_pool.empty()) {
+        hipEvent_t e = h->event_pool.back();
+        h->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets one launch with events when timing is enabled.
+template <class F>
+int timed(wenofd_solver* h, int kind, F&& launch) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (h->timing) {
+        a = take_event(h);
+        b = take_event(h);
+        if (a && b) HIP_TRY(hipEventRecord(a, h->stream));
+    }
+    hipError_t e = launch();
+    if (e != hipSuccess) return fail(WENOFD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    if (h->timing && a && b) {
+        HIP_TRY(hipEventRecord(b, h->stream));
+        h->events.push_back({kind, a, b});
+    }
+    return 0;
+}
+
+int collect_timing(wenofd_solver* h) {
+    if (h->events.empty()) return 0;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (auto& ev : h->events) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ev.start, ev.stop));
+        h->t_ms[ev.kind] += ms;
+        h->t_cnt[ev.kind] += 1;
+        h->event_pool.push_back(ev.start);
+        h->event_pool.push_back(ev.stop);
+    }
+    h->events.clear();
+    return 0;
+}
+
+int set_dev(wenofd_solver* h) {
+    HIP_TRY(hipSetDevice(h->device));
+    return 0;
+}
+
+int fill_ghosts(wenofd_solver* h) {
+    return timed(h, WENOFD_K_GHOST, [&] { return launch_ghost_fill(h->U, h->special, h->g, h->bc, h->stream); });
+}
+
+// One RK stage on the current state: ghost fill, x and y face fluxes, fused
+// rhs + update.
+int stage(wenofd_solver* h, const RkCoef& k, double* rhs_out) {
+    int rc;
+    if ((rc = fill_ghosts(h))) return rc;
+    if ((rc = timed(h, WENOFD_K_FLUX_X,
+                    [&] { return launch_flux_x(h->opt.scheme, h->U, h->Fx, h->g, h->stream); })))
+        return rc;
+    if ((rc = timed(h, WENOFD_K_FLUX_Y,
+                    [&] { return launch_flux_y(h->opt.scheme, h->U, h->Fy, h->g, h->stream); })))
+        return rc;
+    return timed(h, WENOFD_K_RK,
+                 [&] { return launch_rk_update(h->U, h->Un, h->Fx, h->Fy, rhs_out, h->g, k, h->stream); });
+}
+
+int run_rk(wenofd_solver* h, int rkmeth, double dt) {
+    const int grav = h->opt.testcase == kRTI ? 1 : 0;
+    int rc;
+    if (rkmeth == 0) {  // RunRK1, src/WENOFD.cpp:301-311
+        RkCoef k{kRk1, grav, 0.0, 1.0, dt};
+        return stage(h, k, nullptr);
+    }
+    if (rkmeth == 1) {  // RunRK2, :312-351
+        RkCoef k1{kRkFirst, grav, 1.0, 0.0, 1.0 * dt};
+        RkCoef k2{kRkStage, grav, 0.5, 0.5, 0.5 * dt};
+        if ((rc = stage(h, k1, nullptr))) return rc;
+        return stage(h, k2, nullptr);
+    }
+    if (rkmeth == 2) {  // RunRK3, :352-396
+        RkCoef k1{kRkFirst, grav, 1.0, 0.0, 1.0 * dt};
+        RkCoef k2{kRkStage, grav, 3.0 / 4.0, 1.0 / 4.0, (1.0 / 4.0) * dt};
+        RkCoef k3{kRkStage, grav, 1.0 / 3.0, 2.0 / 3.0, (2.0 / 3.0) * dt};
+        if ((rc = stage(h, k1, nullptr))) return rc;
+        if ((rc = stage(h, k2, nullptr))) return rc;
+        return stage(h, k3, nullptr);
+    }
+    return fail(WENOFD_E_BAD_ENUM, "RKMETH must be 0, 1 or 2");
+}
+
+int delta_t(wenofd_solver* h, double* dt) {
+    int rc = timed(h, WENOFD_K_DT, [&] {
+        return launch_dt(h->U, h->g, h->opt.cfl, h->dt_part, h->dt_part_nan, kDtBlocks, h->dt_dev, h->stream);
+    });
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(h->dt_host, h->dt_dev, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    double ts = h->dt_host[0];
+    if (h->dt_host[1] != 0.0) {
+        *dt = NAN;
+        return fail(WENOFD_E_NAN_DT, "Timestep is not a number (non-physical state)");
+    }
+    if (h->cs.t_end - h->now < ts) ts = h->cs.t_end - h->now;  // :434-435
+    *dt = ts;
+    return 0;
+}
+
+// Host copy of the interior state in the reference order [nx][ny][4].
+int download_ref(wenofd_solver* h, std::vector<double>& out) {
+    std::vector<double> full(h->state_doubles());
+    HIP_TRY(hipMemcpyAsync(full.data(), h->U, full.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const int nx = h->g.nx, ny = h->g.ny;
+    out.resize((size_t)nx * ny * 4);
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int r = 0; r < 4; ++r)
+                out[((size_t)i * ny + j) * 4 + r] = full[r * h->g.plane + (size_t)(j + 4) * h->g.pitch + (i + 4)];
+    return 0;
+}
+
+int upload_initial(wenofd_solver* h) {
+    // initializeAve (src/WENOFD.cpp:165-188) on the host, interior only.
+    const int nx = h->g.nx, ny = h->g.ny;
+    std::vector<double> full(h->state_doubles(), 0.0);
+    for (int i = 4; i < nx + 4; ++i)
+        for (int j = 4; j < ny + 4; ++j) {
+            double Uc[4];
+            h->cs.conserved0(h->xc[i], h->yc[j], Uc);
+            for (int r = 0; r < 4; ++r) full[r * h->g.plane + (size_t)j * h->g.pitch + i] = Uc[r];
+        }
+    HIP_TRY(hipMemcpyAsync(h->U, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->Un, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->now = 0;
+    h->steps = 0;
+    return 0;
+}
+
+int create_impl(const std::string& text, int device, wenofd_solver** out) {
+    std::map<std::string, std::string> kv;
+    parse_config_lines(text, kv);
+    Options o;
+    std::string err;
+    int rc = interpret_options(kv, o, err);
+    if (rc) return fail(rc, err);
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(WENOFD_E_HIP, "no HIP device available");
+    if (device < 0) HIP_TRY(hipGetDevice(&device));
+    if (device >= ndev) return fail(WENOFD_E_ARG, "device index out of range");
+
+    auto* h = new wenofd_solver();
+    h->opt = o;
+    h->device = device;
+    make_case(o, h->cs);
+    Geom& g = h->g;
+    g.nx = o.nx;
+    g.ny = o.ny;
+    auto round_up = [](long v, long m) { return (v + m - 1) / m * m; };
+    g.pitch = (int)round_up(g.nx + 8, 32);
+    g.plane = (long)g.pitch * (g.ny + 8);
+    g.fx_pitch = (int)round_up(g.nx + 1, 32);
+    g.fx_plane = (long)g.fx_pitch * g.ny;
+    g.fy_pitch = (int)round_up(g.nx, 32);
+    g.fy_plane = (long)g.fy_pitch * (g.ny + 1);
+    // grid conventions, src/WENOFD.cpp:73-106 (one rank: procXLeft = xL + 0 * dx)
+    g.dx = (h->cs.xR - h->cs.xL) / (g.nx - 1);
+    g.dy = (h->cs.yR - h->cs.yL) / (g.ny - 1);
+    g.gamma = h->cs.gamma;
+    g.zp_lambda = std::pow(g.dx, 2.0 / 3.0);  // include/Utility.hpp:531 with m_deltaX (:836)
+    const double gxl = (h->cs.xL + 0 * g.dx) - g.dx * 4;
+    const double gyl = h->cs.yL - g.dy * 4;
+    h->xc.resize(g.nx + 8);
+    h->yc.resize(g.ny + 8);
+    for (int i = 0; i < g.nx + 8; ++i) h->xc[i] = gxl + i * g.dx;
+    for (int j = 0; j < g.ny + 8; ++j) h->yc[j] = gyl + j * g.dy;
+    h->bc = Bc{h->cs.bc_left, h->cs.bc_right, h->cs.bc_bottom};
+
+    auto cleanup_fail = [&](int code) {
+        wenofd_destroy(h);
+        return code;
+    };
+    if ((rc = set_dev(h))) return cleanup_fail(rc);
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&h->U, h->state_doubles() * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&h->Un, h->state_doubles() * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&h->Fx, 4 * (size_t)g.fx_plane * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&h->Fy, 4 * (size_t)g.fy_plane * sizeof(double));
+    const int ns = 2 * (g.nx + g.ny);
+    if (e == hipSuccess) e = hipMalloc(&h->special, 4 * (size_t)ns * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&h->dt_part, kDtBlocks * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&h->dt_part_nan, kDtBlocks * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&h->dt_dev, 2 * sizeof(double));
+    if (e == hipSuccess) e = hipHostMalloc(&h->dt_host, 2 * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) return cleanup_fail(fail(WENOFD_E_HIP, std::string("allocation: ") + hipGetErrorString(e)));
+
+    // SPECIAL boundary values (time-invariant): getU0 at the boundary point of
+    // each column / row (src/WENOFD.cpp:1045-1058, 1092-1102, 1138-1149).
+    std::vector<double> sp(4 * (size_t)ns, 0.0);
+    const int sx = 4, ex = g.nx + 4, sy = 4, ey = g.ny + 4;
+    for (int i = 0; i < g.nx; ++i) {
+        double Ub[4], Ut[4];
+        h->cs.conserved0(h->xc[sx + i], h->yc[sy], Ub);
+        h->cs.conserved0(h->xc[sx + i], h->yc[ey - 1], Ut);
+        for (int r = 0; r < 4; ++r) {
+            sp[r * ns + i] = Ub[r];
+            sp[r * ns + g.nx + i] = Ut[r];
+        }
+    }
+    for (int j = 0; j < g.ny; ++j) {
+        double Ul[4], Ur[4];
+        h->cs.conserved0(h->xc[sx], h->yc[sy + j], Ul);
+        h->cs.conserved0(h->xc[ex], h->yc[sy + j], Ur);  // first ghost column's coordinate (:1144)
+        for (int r = 0; r < 4; ++r) {
+            sp[r * ns + 2 * g.nx + j] = Ul[r];
+            sp[r * ns + 2 * g.nx + g.ny + j] = Ur[r];
+        }
+    }
+    e = hipMemcpy(h->special, sp.data(), sp.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(h->Fx, 0, 4 * (size_t)g.fx_plane * sizeof(double), h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->Fy, 0, 4 * (size_t)g.fy_plane * sizeof(double), h->stream);
+    if (e != hipSuccess) return cleanup_fail(fail(WENOFD_E_HIP, std::string("upload: ") + hipGetErrorString(e)));
+    if ((rc = upload_initial(h))) return cleanup_fail(rc);
+    *out = h;
+    return 0;
+}
+
+std::string timestamp() {
+    // utils/timer.hpp:81-88
+    auto now = std::chrono::system_clock::now();
+    std::time_t t = std::chrono::system_clock::to_time_t(now);
+    std::tm tmv;
+    localtime_r(&t, &tmv);
+    char buf[32];
+    std::strftime(buf, sizeof buf, "%Y%m%d_%H%M%S", &tmv);
+    return buf;
+}
+
+int mkdirs(const std::string& path) {
+    std::string cur;
+    for (size_t k = 0; k < path.size(); ++k) {
+        cur += path[k];
+        if (path[k] == '/' || k + 1 == path.size()) {
+            if (cur == "/" || cur == "./" || cur == ".") continue;
+            struct stat st;
+            if (stat(cur.c_str(), &st) != 0 && mkdir(cur.c_str(), 0755) != 0) return -1;
+        }
+    }
+    return 0;
+}
+
+int ensure_out_dir(wenofd_solver* h) {
+    if (h->out_dir.empty()) {
+        // src/WENOFD.cpp:83-86
+        h->out_dir = h->opt.output_dir + timestamp() + "_" + kCaseNames[h->opt.testcase] + "_" +
+                     kSchemeNames[h->opt.scheme] + "_" + std::to_string(h->g.nx) + "x" + std::to_string(h->g.ny) +
+                     "/";
+    }
+    if (mkdirs(h->out_dir)) return fail(WENOFD_E_IO, "cannot create " + h->out_dir);
+    return 0;
+}
+
+void norms(wenofd_solver* h, const std::vector<double>& Uref, double* out3) {
+    // outputAccuracy (src/WENOFD.cpp:1308-1360): loop order x outer, y inner
+    const int nx = h->g.nx, ny = h->g.ny;
+    double e1 = 0, e2 = 0, einf = 0;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            const double ex = h->cs.exact_rho(h->xc[i + 4], h->yc[j + 4], h->now);
+            const double e = std::fabs(ex - Uref[((size_t)i * ny + j) * 4]);
+            e1 += e;
+            e2 += e * e;
+            if (e > einf) einf = e;
+        }
+    out3[0] = einf;
+    out3[1] = e1 / nx / ny;
+    out3[2] = std::sqrt(e2 / nx / ny);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* wenofd_last_error(void) { return g_last_error.c_str(); }
+
+int wenofd_version(void) { return 100; }
+
+int wenofd_parse_config(const char* config_text, wenofd_info* info) {
+    if (!config_text || !info) return fail(WENOFD_E_ARG, "null argument");
+    std::map<std::string, std::string> kv;
+    parse_config_lines(config_text, kv);
+    Options o;
+    std::string err;
+    int rc = interpret_options(kv, o, err);
+    if (rc) return fail(rc, err);
+    CaseDef c;
+    make_case(o, c);
+    std::memset(info, 0, sizeof *info);
+    info->cfl = o.cfl;
+    info->xL = c.xL; info->xR = c.xR; info->yL = c.yL; info->yR = c.yR;
+    info->dx = (c.xR - c.xL) / (o.nx - 1);
+    info->dy = (c.yR - c.yL) / (o.ny - 1);
+    info->t_end = c.t_end;
+    info->gamma = c.gamma;
+    info->zp_lambda = std::pow(info->dx, 2.0 / 3.0);
+    info->nx = o.nx; info->ny = o.ny;
+    info->testcase = o.testcase; info->scheme = o.scheme; info->rkmeth = o.rkmeth;
+    info->device = -1;
+    return 0;
+}
+
+int wenofd_create(const char* config_text, int device, wenofd_solver** out) {
+    if (!config_text || !out) return fail(WENOFD_E_ARG, "null argument");
+    *out = nullptr;
+    return create_impl(config_text, device, out);
+}
+
+int wenofd_create_from_file(const char* path, int device, wenofd_solver** out) {
+    if (!path || !out) return fail(WENOFD_E_ARG, "null argument");
+    *out = nullptr;
+    std::ifstream in(path);
+    if (!in.is_open()) return fail(WENOFD_E_CONFIG, std::string("Failed to open file") + path);  // readConfig.hpp:44-48
+    std::stringstream ss;
+    ss << in.rdbuf();
+    return create_impl(ss.str(), device, out);
+}
+
+int wenofd_destroy(wenofd_solver* h) {
+    if (!h) return 0;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto& ev : h->events) {
+        (void)hipEventDestroy(ev.start);
+        (void)hipEventDestroy(ev.stop);
+    }
+    for (auto e : h->event_pool) (void)hipEventDestroy(e);
+    (void)hipFree(h->U);
+    (void)hipFree(h->Un);
+    (void)hipFree(h->Fx);
+    (void)hipFree(h->Fy);
+    (void)hipFree(h->rhs);
+    (void)hipFree(h->special);
+    (void)hipFree(h->dt_part);
+    (void)hipFree(h->dt_part_nan);
+    (void)hipFree(h->dt_dev);
+    if (h->dt_host) (void)hipHostFree(h->dt_host);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+int wenofd_initialize_ave(wenofd_solver* h) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    int rc = set_dev(h);
+    return rc ? rc : upload_initial(h);
+}
+
+int wenofd_calculate_delta_t(wenofd_solver* h, double* dt) {
+    if (!h || !dt) return fail(WENOFD_E_ARG, "null argument");
+    int rc = set_dev(h);
+    return rc ? rc : delta_t(h, dt);
+}
+
+int wenofd_run_rk(wenofd_solver* h, int rkmeth, double dt) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    int rc = set_dev(h);
+    return rc ? rc : run_rk(h, rkmeth, dt);
+}
+
+int wenofd_set_boundary(wenofd_solver* h) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    int rc = set_dev(h);
+    return rc ? rc : fill_ghosts(h);
+}
+
+int wenofd_get_flux(wenofd_solver* h) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    int rc = set_dev(h);
+    if (rc) return rc;
+    if (!h->rhs) {
+        HIP_TRY(hipMalloc(&h->rhs, h->state_doubles() * sizeof(double)));
+        HIP_TRY(hipMemsetAsync(h->rhs, 0, h->state_doubles() * sizeof(double), h->stream));
+    }
+    RkCoef k{kRkNone, h->opt.testcase == kRTI ? 1 : 0, 0.0, 0.0, 0.0};
+    return stage(h, k, h->rhs);
+}
+
+static int copy_out(wenofd_solver* h, const double* dev, double* host, int layout) {
+    std::vector<double> full(h->state_doubles());
+    HIP_TRY(hipMemcpyAsync(full.data(), dev, full.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const int nx = h->g.nx, ny = h->g.ny;
+    const Geom& g = h->g;
+    if (layout == WENOFD_LAYOUT_REF) {
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j)
+                for (int r = 0; r < 4; ++r)
+                    host[((size_t)i * ny + j) * 4 + r] = full[r * g.plane + (size_t)(j + 4) * g.pitch + (i + 4)];
+    } else if (layout == WENOFD_LAYOUT_SOA) {
+        for (int r = 0; r < 4; ++r)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                    host[((size_t)r * ny + j) * nx + i] = full[r * g.plane + (size_t)(j + 4) * g.pitch + (i + 4)];
+    } else if (layout == WENOFD_LAYOUT_REF_GHOSTED) {
+        for (int i = 0; i < nx + 8; ++i)
+            for (int j = 0; j < ny + 8; ++j) {
+                const bool corner = (i < 4 || i >= nx + 4) && (j < 4 || j >= ny + 4);
+                for (int r = 0; r < 4; ++r)
+                    host[((size_t)i * (ny + 8) + j) * 4 + r] =
+                        corner ? 0.0 : full[r * g.plane + (size_t)j * g.pitch + i];
+            }
+    } else {
+        return fail(WENOFD_E_ARG, "unknown layout");
+    }
+    return 0;
+}
+
+int wenofd_get_rhs(wenofd_solver* h, double* host, int layout) {
+    if (!h || !host) return fail(WENOFD_E_ARG, "null argument");
+    if (!h->rhs) return fail(WENOFD_E_ARG, "no rhs: call wenofd_get_flux first");
+    if (layout == WENOFD_LAYOUT_REF_GHOSTED) return fail(WENOFD_E_ARG, "rhs has no ghost points");
+    int rc = set_dev(h);
+    return rc ? rc : copy_out(h, h->rhs, host, layout);
+}
+
+int wenofd_get_state(wenofd_solver* h, double* host, int layout) {
+    if (!h || !host) return fail(WENOFD_E_ARG, "null argument");
+    int rc = set_dev(h);
+    return rc ? rc : copy_out(h, h->U, host, layout);
+}
+
+int wenofd_set_state(wenofd_solver* h, const double* host, int layout) {
+    if (!h || !host) return fail(WENOFD_E_ARG, "null argument");
+    int rc = set_dev(h);
+    if (rc) return rc;
+    std::vector<double> full(h->state_doubles());
+    HIP_TRY(hipMemcpyAsync(full.data(), h->U, full.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const int nx = h->g.nx, ny = h->g.ny;
+    const Geom& g = h->g;
+    if (layout == WENOFD_LAYOUT_REF) {
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j)
+                for (int r = 0; r < 4; ++r)
+                    full[r * g.plane + (size_t)(j + 4) * g.pitch + (i + 4)] = host[((size_t)i * ny + j) * 4 + r];
+    } else if (layout == WENOFD_LAYOUT_SOA) {
+        for (int r = 0; r < 4; ++r)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                    full[r * g.plane + (size_t)(j + 4) * g.pitch + (i + 4)] = host[((size_t)r * ny + j) * nx + i];
+    } else if (layout == WENOFD_LAYOUT_REF_GHOSTED) {
+        for (int i = 0; i < nx + 8; ++i)
+            for (int j = 0; j < ny + 8; ++j)
+                for (int r = 0; r < 4; ++r)
+                    full[r * g.plane + (size_t)j * g.pitch + i] = host[((size_t)i * (ny + 8) + j) * 4 + r];
+    } else {
+        return fail(WENOFD_E_ARG, "unknown layout");
+    }
+    HIP_TRY(hipMemcpyAsync(h->U, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int wenofd_step(wenofd_solver* h, double dt_fixed, double* dt_used) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    int rc = set_dev(h);
+    if (rc) return rc;
+    double dt = dt_fixed;
+    if (!(dt_fixed > 0) && (rc = delta_t(h, &dt))) return rc;
+    if ((rc = run_rk(h, h->opt.rkmeth, dt))) return rc;
+    h->now += dt;
+    h->steps += 1;
+    if (dt_used) *dt_used = dt;
+    return 0;
+}
+
+int wenofd_advance(wenofd_solver* h, int max_steps, double dt_fixed, int* steps_done) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    int n = 0, rc = 0;
+    while (std::fabs(h->now - h->cs.t_end) > 1e-9) {  // src/WENOFD.cpp:213
+        if (max_steps > 0 && n >= max_steps) break;
+        if ((rc = wenofd_step(h, dt_fixed, nullptr))) break;
+        ++n;
+    }
+    if (steps_done) *steps_done = n;
+    if (rc) return rc;
+    if (h->timing) {
+        // keep the event list bounded during long timed runs
+        if (h->events.size() > 4096 && (rc = collect_timing(h))) return rc;
+    }
+    return 0;
+}
+
+int wenofd_get_info(wenofd_solver* h, wenofd_info* info) {
+    if (!h || !info) return fail(WENOFD_E_ARG, "null argument");
+    std::memset(info, 0, sizeof *info);
+    info->cfl = h->opt.cfl;
+    info->dx = h->g.dx; info->dy = h->g.dy;
+    info->xL = h->cs.xL; info->xR = h->cs.xR; info->yL = h->cs.yL; info->yR = h->cs.yR;
+    info->t_end = h->cs.t_end;
+    info->now = h->now;
+    info->gamma = h->cs.gamma;
+    info->zp_lambda = h->g.zp_lambda;
+    info->steps = h->steps;
+    info->nx = h->g.nx; info->ny = h->g.ny;
+    info->testcase = h->opt.testcase; info->scheme = h->opt.scheme; info->rkmeth = h->opt.rkmeth;
+    info->device = h->device;
+    return 0;
+}
+
+int wenofd_set_time(wenofd_solver* h, double now) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    h->now = now;
+    return 0;
+}
+
+int wenofd_output_dir(wenofd_solver* h, char* buf, int buflen) {
+    if (!h || !buf || buflen <= 0) return fail(WENOFD_E_ARG, "null argument");
+    int rc = ensure_out_dir(h);
+    if (rc) return rc;
+    std::snprintf(buf, buflen, "%s", h->out_dir.c_str());
+    return 0;
+}
+
+int wenofd_output_ave(wenofd_solver* h, const char* prefix) {
+    // outputAve, src/WENOFD.cpp:1161-1238: Tecplot ASCII POINT, x outer, y
+    // inner, default stream formatting (6 significant digits).
+    if (!h || !prefix) return fail(WENOFD_E_ARG, "null argument");
+    int rc = set_dev(h);
+    if (rc || (rc = ensure_out_dir(h))) return rc;
+    std::vector<double> U;
+    if ((rc = download_ref(h, U))) return rc;
+    const std::string fn = h->out_dir + "solution_" + prefix + ".plt";
+    FILE* f = std::fopen(fn.c_str(), "w");
+    if (!f) return fail(WENOFD_E_IO, "cannot write " + fn);
+    std::fprintf(f, "TITLE=DGSolution\nVARIABLES=X Y rho u v pre c E \nZONE T=TA, I=%d, J=%d, DATAPACKING=POINT\n",
+                 h->g.ny, h->g.nx);
+    const double gm = h->cs.gamma;
+    for (int i = 0; i < h->g.nx; ++i)
+        for (int j = 0; j < h->g.ny; ++j) {
+            const double* q = &U[((size_t)i * h->g.ny + j) * 4];
+            const double rho = q[0], u = q[1] / q[0], v = q[2] / q[0], E = q[3];
+            const double pre = (gm - 1.0) * (E - 0.5 * rho * (u * u + v * v));  // getVitalVarVal, equations.hpp:683-698
+            const double c = std::sqrt(gm * pre / rho);
+            std::fprintf(f, "%g %g %g %g %g %g %g %g \n", h->xc[i + 4], h->yc[j + 4], rho, u, v, pre, c, E);
+        }
+    std::fclose(f);
+    return 0;
+}
+
+int wenofd_output_error(wenofd_solver* h, const char* prefix) {
+    // outputError, src/WENOFD.cpp:1239-1306
+    if (!h || !prefix) return fail(WENOFD_E_ARG, "null argument");
+    int rc = set_dev(h);
+    if (rc || (rc = ensure_out_dir(h))) return rc;
+    std::vector<double> U;
+    if ((rc = download_ref(h, U))) return rc;
+    const std::string fn = h->out_dir + "error_" + prefix + ".plt";
+    FILE* f = std::fopen(fn.c_str(), "w");
+    if (!f) return fail(WENOFD_E_IO, "cannot write " + fn);
+    std::fprintf(f, "TITLE=DGSolution\nVARIABLES=X Y error\nZONE T=TA, I=%d, J=%d, DATAPACKING = POINT\n", h->g.ny,
+                 h->g.nx);
+    for (int i = 0; i < h->g.nx; ++i)
+        for (int j = 0; j < h->g.ny; ++j) {
+            const double ex = h->cs.exact_rho(h->xc[i + 4], h->yc[j + 4], h->now);
+            std::fprintf(f, "%g %g %g \n", h->xc[i + 4], h->yc[j + 4],
+                         std::fabs(ex - U[((size_t)i * h->g.ny + j) * 4]));
+        }
+    std::fclose(f);
+    return 0;
+}
+
+int wenofd_accuracy(wenofd_solver* h, double* norms3) {
+    if (!h || !norms3) return fail(WENOFD_E_ARG, "null argument");
+    int rc = set_dev(h);
+    if (rc) return rc;
+    std::vector<double> U;
+    if ((rc = download_ref(h, U))) return rc;
+    norms(h, U, norms3);
+    return 0;
+}
+
+int wenofd_output_accuracy(wenofd_solver* h, const char* prefix, double* norms3) {
+    // outputAccuracy, src/WENOFD.cpp:1308-1382
+    if (!h || !prefix) return fail(WENOFD_E_ARG, "null argument");
+    double nv[3];
+    int rc = wenofd_accuracy(h, nv);
+    if (rc || (rc = ensure_out_dir(h))) return rc;
+    const std::string fn = h->out_dir + "accuracy_" + prefix + ".csv";
+    FILE* f = std::fopen(fn.c_str(), "w");
+    if (!f) return fail(WENOFD_E_IO, "cannot write " + fn);
+    std::fprintf(f, "elemNum, Linf-norm (rho), L1-norm (rho), L2-norm (rho)\n%d, %20.15e, %20.15e, %20.15e", h->g.nx,
+                 nv[0], nv[1], nv[2]);
+    std::fclose(f);
+    if (norms3) std::memcpy(norms3, nv, sizeof nv);
+    return 0;
+}
+
+int wenofd_run(wenofd_solver* h, int verbose) {
+    // CWENOFD::run, src/WENOFD.cpp:189-299
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    int rc;
+    if ((rc = wenofd_output_ave(h, "initial"))) return rc;
+    if (verbose) std::printf("Start iteration...\n");
+    const auto t0 = std::chrono::steady_clock::now();
+    int count = 0;
+    while (std::fabs(h->now - h->cs.t_end) > 1e-9) {
+        double dt = 0;
+        if ((rc = wenofd_step(h, 0.0, &dt))) return rc;
+        ++count;
+        if (verbose && count % 20 == 1) {
+            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            const double progress = h->now / h->cs.t_end;
+            const int bar = 50, pos = (int)(progress * bar);
+            std::printf("Iteration count: %d, DeltaT: %.2e, Elapsed time: %.2f seconds, Progress: %.2f/%.2f (%.2f%%) [",
+                        count, dt, el, h->now, h->cs.t_end, progress * 100.0);
+            for (int k = 0; k < bar; ++k) std::putchar(k < pos ? '=' : (k == pos ? '>' : ' '));
+            std::printf("]\n");
+            std::fflush(stdout);
+        }
+    }
+    if ((rc = wenofd_output_ave(h, "final"))) return rc;
+    if (h->cs.has_exact) {
+        if ((rc = wenofd_output_accuracy(h, "final", nullptr))) return rc;
+        if ((rc = wenofd_output_error(h, "final"))) return rc;
+    }
+    return 0;
+}
+
+int wenofd_set_timing(wenofd_solver* h, int enable) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    h->timing = enable != 0;
+    return 0;
+}
+
+int wenofd_get_timing(wenofd_solver* h, double* ms, long long* counts, int nkinds) {
+    if (!h || !ms || !counts || nkinds <= 0) return fail(WENOFD_E_ARG, "null argument");
+    int rc = set_dev(h);
+    if (rc || (rc = collect_timing(h))) return rc;
+    for (int k = 0; k < nkinds && k < WENOFD_K_COUNT; ++k) {
+        ms[k] = h->t_ms[k];
+        counts[k] = h->t_cnt[k];
+        h->t_ms[k] = 0;
+        h->t_cnt[k] = 0;
+    }
+    return 0;
+}
+
+int wenofd_synchronize(wenofd_solver* h) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    int rc = set_dev(h);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int wenofd_probe_weno5(int device, int scheme, double zp_lambda, int n, const double* v5, double* out) {
+    if (!v5 || !out || n < 0) return fail(WENOFD_E_ARG, "bad argument");
+    if (scheme < 0 || scheme > 3) return fail(WENOFD_E_BAD_ENUM, "SCHEME must be 0..3");
+    if (n == 0) return 0;
+    if (device >= 0) HIP_TRY(hipSetDevice(device));
+    double *dv = nullptr, *dout = nullptr;
+    HIP_TRY(hipMalloc(&dv, 5 * (size_t)n * sizeof(double)));
+    hipError_t e = hipMalloc(&dout, (size_t)n * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(dv, v5, 5 * (size_t)n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_probe_weno(scheme, dv, n, zp_lambda, dout, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(dv);
+    (void)hipFree(dout);
+    if (e != hipSuccess) return fail(WENOFD_E_HIP, std::string("probe_weno5: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int wenofd_probe_physics(int device, double gamma, int n, const double* states, double* out82) {
+    if (!states || !out82 || n < 0) return fail(WENOFD_E_ARG, "bad argument");
+    if (n == 0) return 0;
+    if (device >= 0) HIP_TRY(hipSetDevice(device));
+    double *du = nullptr, *dout = nullptr;
+    HIP_TRY(hipMalloc(&du, 4 * (size_t)n * sizeof(double)));
+    hipError_t e = hipMalloc(&dout, (size_t)n * kProbePhysStride * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(du, states, 4 * (size_t)n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_probe_physics(du, n, gamma, dout, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out82, dout, (size_t)n * kProbePhysStride * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(du);
+    (void)hipFree(dout);
+    if (e != hipSuccess) return fail(WENOFD_E_HIP, std::string("probe_physics: ") + hipGetErrorString(e));
+    return 0;
+}
+
+}  // extern "C"

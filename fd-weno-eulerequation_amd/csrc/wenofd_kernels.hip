@@ -1,0 +1,462 @@
+// wenofd_kernels.hip — HIP kernels of the WENO-FD hot path for gfx950 (MI355X).
+//
+// State layout in HBM (one solve): structure of arrays, x fastest,
+//     U[v * plane + j * pitch + i],  v in 0..3 (rho, rho u, rho v, E),
+//     i in [0, nx+8) along x, j in [0, ny+8) along y, 4 ghost points per side,
+// interior i, j in [4, n+4) (the reference's node numbering, src/WENOFD.cpp:36-70).
+// Face fluxes: Fx[v][j-4][f-4] for faces f in [4, nx+4] of interior rows
+// (face f sits between points f-1 and f), Fy[v][f-4][i-4] likewise along y.
+//
+// One RK stage = ghost fill -> x faces -> y faces -> fused rhs + RK update.
+// All arithmetic is FP64 in the reference's operation order (wenofd_math.hpp).
+
+#include <hip/hip_runtime.h>
+
+#include "wenofd_kernels.hpp"
+#include "wenofd_math.hpp"
+
+namespace wenofd {
+
+// ---------------------------------------------------------------------------
+// Ghost fill: exchangeGhostCellsValue (one-rank periodic-x ring) + setBoundary,
+// src/WENOFD.cpp:850-929 and :998-1159, restricted to the ghost points the
+// sweeps read (x ghosts of interior rows, y ghosts of interior columns; corner
+// ghosts are never read). Pure copies and sign flips: bitwise.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ghost_fill(double* __restrict__ U, const double* __restrict__ special,
+                                                    Geom g, Bc bc) {
+    const int nxg = 8 * g.ny;  // x-ghost points: 4 left + 4 right per interior row
+    const int nyg = 8 * g.nx;  // y-ghost points
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nxg + nyg) return;
+    const int sx = 4, ex = g.nx + 4, sy = 4, ey = g.ny + 4;
+    int di, dj, si, sj, flip = -1, spec = -1;
+    if (k < nxg) {
+        const int j = sy + k / 8;
+        const int e = k % 8;
+        dj = sj = j;
+        if (e < 4) {  // left ghost i = sx - e1
+            const int e1 = e + 1;
+            di = sx - e1;
+            if (bc.left == kBcPeriod) si = ex - 1 - e1;
+            else if (bc.left == kBcSpecial) { si = di; spec = 2 * g.nx + (j - sy); }
+            else { si = sx + e1; if (bc.left == kBcSlip) flip = 1; }
+        } else {  // right ghost i = ex - 1 + e1
+            const int e1 = e - 3;
+            di = ex - 1 + e1;
+            if (bc.right == kBcPeriod) si = sx + e1;
+            else if (bc.right == kBcSpecial) { si = di; spec = 2 * g.nx + g.ny + (j - sy); }
+            else { si = ex - 1 - e1; if (bc.right == kBcSlip) flip = 1; }
+        }
+    } else {
+        const int kk = k - nxg;
+        const int i = sx + kk / 8;
+        const int e = kk % 8;
+        di = si = i;
+        if (e < 4) {  // bottom ghost j = sy - e1 (the bottom boundary type governs both y sides, :1009)
+            const int e1 = e + 1;
+            dj = sy - e1;
+            if (bc.bottom == kBcPeriod) sj = ey - 1 - e1;
+            else if (bc.bottom == kBcSpecial) { sj = dj; spec = i - sx; }
+            else { sj = sy + e1; if (bc.bottom == kBcSlip) flip = 2; }
+        } else {  // top ghost j = ey - 1 + e1
+            const int e1 = e - 3;
+            dj = ey - 1 + e1;
+            if (bc.bottom == kBcPeriod) sj = sy + e1;
+            else if (bc.bottom == kBcSpecial) { sj = dj; spec = g.nx + (i - sx); }
+            else { sj = ey - 1 - e1; if (bc.bottom == kBcSlip) flip = 2; }
+        }
+    }
+    const size_t dst = (size_t)dj * g.pitch + di;
+    if (spec >= 0) {
+        const int ns = 2 * (g.nx + g.ny);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) U[r * g.plane + dst] = special[r * ns + spec];
+        return;
+    }
+    const size_t src = (size_t)sj * g.pitch + si;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const double val = U[r * g.plane + src];
+        U[r * g.plane + dst] = (r == flip) ? -val : val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Characteristic-wise LLF-split WENO5 flux at one face (src/WENOFD.cpp:526-644):
+// arithmetic-mean state -> L, R -> project the 5-point F+ (points f-3..f+1) and
+// F- (points f-2..f+2) stencils -> 2 x 4 WENO5 -> R (v+ + v-).
+// sp/sm(s, r) return F+/F- component r of stencil slot s (slot 0 = point f-3).
+// ---------------------------------------------------------------------------
+template <int D, int S, class GetP, class GetM>
+__device__ __forceinline__ void face_flux(const double Ul[4], const double Ur[4], GetP sp, GetM sm, double gamma,
+                                          double zp_lambda, double Fh[4]) {
+    double ave[4], L[16], R[16];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ave[r] = 0.5 * (Ul[r] + Ur[r]);
+    eigen_matrices<D>(ave, gamma, L, R);
+    double vp[5][4], vm[5][4];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        double fp[4], fm[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            fp[r] = sp(s, r);
+            fm[r] = sm(s + 1, r);
+        }
+        project_left<D>(L, fp, vp[s]);
+        project_left<D>(L, fm, vm[s]);
+    }
+    double vh[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double hp = weno5<S>(vp[0][k], vp[1][k], vp[2][k], vp[3][k], vp[4][k], zp_lambda);
+        const double hm = weno5<S>(vm[4][k], vm[3][k], vm[2][k], vm[1][k], vm[0][k], zp_lambda);
+        vh[k] = hp + hm;
+    }
+    project_right<D>(R, vh, Fh);
+}
+
+// ---------------------------------------------------------------------------
+// x faces. One block = 256 consecutive faces of one interior row; the LLF split
+// of the 261 points they read is computed once into LDS (component-major, so a
+// wave's 64 consecutive slots hit 64 distinct banks), then each thread builds
+// one face flux. Loads of U are coalesced along x.
+// ---------------------------------------------------------------------------
+constexpr int kXT = 256;           // faces per block
+constexpr int kXP = kXT + 5;       // points per block
+constexpr int kXS = kXT + 8;       // LDS row stride (slots)
+
+template <int S>
+__global__ __launch_bounds__(kXT) void k_flux_x(const double* __restrict__ U, double* __restrict__ Fx, Geom g) {
+    __shared__ double sP[4][kXS];
+    __shared__ double sM[4][kXS];
+    __shared__ double sU[4][kXS];
+    const int t = threadIdx.x;
+    const int j = blockIdx.y + 4;
+    const int f0 = 4 + blockIdx.x * kXT;
+    const double* row = U + (size_t)j * g.pitch;
+    for (int k = t; k < kXP; k += kXT) {
+        const int p = f0 - 3 + k;
+        if (p < g.nx + 8) {
+            double u[4], fm[4], fp[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) u[r] = row[r * g.plane + p];
+            flux_split<0>(u, g.gamma, fm, fp);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                sP[r][k] = fp[r];
+                sM[r][k] = fm[r];
+                sU[r][k] = u[r];
+            }
+        }
+    }
+    __syncthreads();
+    const int f = f0 + t;
+    if (f > g.nx + 4) return;
+    double Ul[4], Ur[4], Fh[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        Ul[r] = sU[r][t + 2];
+        Ur[r] = sU[r][t + 3];
+    }
+    face_flux<0, S>(
+        Ul, Ur, [&](int s, int r) { return sP[r][t + s]; }, [&](int s, int r) { return sM[r][t + s]; }, g.gamma,
+        g.zp_lambda, Fh);
+    const size_t o = (size_t)(j - 4) * g.fx_pitch + (f - 4);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Fx[r * g.fx_plane + o] = Fh[r];
+}
+
+// ---------------------------------------------------------------------------
+// y faces. One block = 64 interior columns x kYF faces along y, marched in
+// steps of 4 face rows (256 threads = 64 columns x 4 rows). The LLF split of
+// the 9 point rows one step reads lives in an LDS ring; each step adds 4 new
+// rows, so every point is split once per block (plus the 5-row lead-in).
+// ---------------------------------------------------------------------------
+constexpr int kYC = 64;    // columns per block
+constexpr int kYR = 4;     // face rows per step
+constexpr int kYF = 64;    // faces per block along y
+constexpr int kRing = 9;   // ring rows: a step reads 9 consecutive point rows
+
+template <int S>
+__global__ __launch_bounds__(kYC* kYR) void k_flux_y(const double* __restrict__ U, double* __restrict__ Fy, Geom g) {
+    __shared__ double sP[4][kRing][kYC];
+    __shared__ double sM[4][kRing][kYC];
+    const int tx = threadIdx.x % kYC;
+    const int ty = threadIdx.x / kYC;
+    const int i = 4 + blockIdx.x * kYC + tx;
+    const bool col_ok = i < g.nx + 4;
+    const int F0 = 4 + blockIdx.y * kYF;                  // first face of the block
+    const int F1 = min(F0 + kYF, g.ny + 5);               // one past the last face
+    const int base = F0 - 3;                              // point row held in ring slot 0 at start
+
+    auto load_row = [&](int prow) {
+        const int slot = (prow - base) % kRing;
+        if (col_ok && prow < g.ny + 8) {
+            double u[4], fm[4], fp[4];
+            const size_t o = (size_t)prow * g.pitch + i;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) u[r] = U[r * g.plane + o];
+            flux_split<1>(u, g.gamma, fm, fp);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                sP[r][slot][tx] = fp[r];
+                sM[r][slot][tx] = fm[r];
+            }
+        }
+    };
+    // lead-in: point rows F0-3 .. F0+1
+    for (int k = ty; k < 5; k += kYR) load_row(base + k);
+    for (int f = F0; f < F1; f += kYR) {
+        load_row(f + 2 + ty);  // rows f+2 .. f+5 complete the window [f-3, f+6)
+        __syncthreads();
+        const int fr = f + ty;
+        if (col_ok && fr < F1) {
+            double Ul[4], Ur[4], Fh[4];
+            const size_t ol = (size_t)(fr - 1) * g.pitch + i;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                Ul[r] = U[r * g.plane + ol];
+                Ur[r] = U[r * g.plane + ol + g.pitch];
+            }
+            const int s0 = fr - 3 - base;  // ring position of point row fr-3
+            face_flux<1, S>(
+                Ul, Ur, [&](int s, int r) { return sP[r][(s0 + s) % kRing][tx]; },
+                [&](int s, int r) { return sM[r][(s0 + s) % kRing][tx]; }, g.gamma, g.zp_lambda, Fh);
+            const size_t o = (size_t)(fr - 4) * g.fy_pitch + (i - 4);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Fy[r * g.fy_plane + o] = Fh[r];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// rhs + RK stage, src/WENOFD.cpp:454-465 (rhs = x term + y term [+ source]),
+// :800-821 (RTI gravity), :301-396 (stage update):
+//   mode kRk1:     U = U + dt rhs
+//   mode kRkStage: U = (a Un + b U) + (c dt) rhs
+//   mode kRkFirst: Un = U, then as kRkStage
+// rhs_out (optional) receives rhs in the state layout (interior only).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_rk_update(double* __restrict__ U, double* __restrict__ Un,
+                                                   const double* __restrict__ Fx, const double* __restrict__ Fy,
+                                                   double* __restrict__ rhs_out, Geom g, RkCoef k) {
+    const int il = blockIdx.x * blockDim.x + threadIdx.x;
+    const int jl = blockIdx.y;
+    if (il >= g.nx) return;
+    const size_t o = (size_t)(jl + 4) * g.pitch + (il + 4);
+    const size_t ox = (size_t)jl * g.fx_pitch + il;
+    const size_t oy = (size_t)jl * g.fy_pitch + il;
+    double rhs[4], u[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const double tx = -(Fx[r * g.fx_plane + ox + 1] - Fx[r * g.fx_plane + ox]) / g.dx;
+        const double ty = -(Fy[r * g.fy_plane + oy + g.fy_pitch] - Fy[r * g.fy_plane + oy]) / g.dy;
+        rhs[r] = tx + ty;
+        u[r] = U[r * g.plane + o];
+    }
+    if (k.gravity) {
+        rhs[3] = rhs[3] + u[2];
+        rhs[2] = rhs[2] + u[0];
+    }
+    if (rhs_out) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) rhs_out[r * g.plane + o] = rhs[r];
+    }
+    if (k.mode == kRkNone) return;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        double nu;
+        if (k.mode == kRk1) {
+            nu = u[r] + k.cdt * rhs[r];
+        } else {
+            double un;
+            if (k.mode == kRkFirst) {
+                un = u[r];
+                Un[r * g.plane + o] = un;
+            } else {
+                un = Un[r * g.plane + o];
+            }
+            nu = (k.a * un + k.b * u[r]) + k.cdt * rhs[r];
+        }
+        U[r * g.plane + o] = nu;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// calculateDeltaT, src/WENOFD.cpp:398-452 with utils/equations.hpp:137-147:
+// per interior point cfl / ((|u|+c)/dx + (|v|+c)/dy), minimum over the domain.
+// Two passes: per-block minimum (wavefront shuffles, then LDS), then one block
+// folds the partials. MIN is exact, so the result does not depend on order.
+// A NaN anywhere raises the flag (the reference aborts on a NaN step).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void min_nan_reduce_block(double& m, int& nan, double* sm, int* sn) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        const int on = __shfl_xor(nan, off, 64);
+        m = (o < m) ? o : m;
+        nan |= on;
+    }
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64, nw = blockDim.x / 64;
+    if (lane == 0) {
+        sm[w] = m;
+        sn[w] = nan;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < nw; ++q) {
+            m = (sm[q] < m) ? sm[q] : m;
+            nan |= sn[q];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dt_partial(const double* __restrict__ U, Geom g, double cfl,
+                                                    double* __restrict__ part, int* __restrict__ part_nan) {
+    __shared__ double sm[4];
+    __shared__ int sn[4];
+    double m = 1.0;
+    int nan = 0;
+    const long total = (long)g.nx * g.ny;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (long)gridDim.x * blockDim.x) {
+        const int jl = (int)(k / g.nx), il = (int)(k % g.nx);
+        const size_t o = (size_t)(jl + 4) * g.pitch + (il + 4);
+        const Prim q = primitive(U[o], U[g.plane + o], U[2 * g.plane + o], U[3 * g.plane + o], g.gamma);
+        const double ex = fabs(q.u) + q.c;
+        const double ey = fabs(q.v) + q.c;
+        const double tmp = cfl / (ex / g.dx + ey / g.dy);
+        if (tmp != tmp) nan = 1;
+        m = (tmp < m) ? tmp : m;
+    }
+    min_nan_reduce_block(m, nan, sm, sn);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = m;
+        part_nan[blockIdx.x] = nan;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dt_final(const double* __restrict__ part, const int* __restrict__ part_nan,
+                                                  int n, double* __restrict__ out) {
+    __shared__ double sm[4];
+    __shared__ int sn[4];
+    double m = 1.0;
+    int nan = 0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        m = (part[k] < m) ? part[k] : m;
+        nan |= part_nan[k];
+    }
+    min_nan_reduce_block(m, nan, sm, sn);
+    if (threadIdx.x == 0) {
+        out[0] = m;
+        out[1] = nan ? 1.0 : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Probes: the same device functions over independent inputs (used by the
+// per-function parity tests through the C ABI).
+// ---------------------------------------------------------------------------
+template <int S>
+__global__ void k_probe_weno(const double* __restrict__ v5, int n, double zp_lambda, double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double* v = v5 + 5 * (size_t)k;
+    out[k] = weno5<S>(v[0], v[1], v[2], v[3], v[4], zp_lambda);
+}
+
+// per state: Lx(16) Rx(16) Ly(16) Ry(16) Fm_x Fp_x Fm_y Fp_y (4 each) maxeig_x maxeig_y
+__global__ void k_probe_physics(const double* __restrict__ Uin, int n, double gamma, double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double U[4], L[16], R[16], fm[4], fp[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) U[r] = Uin[4 * (size_t)k + r];
+    double* o = out + (size_t)k * kProbePhysStride;
+    eigen_matrices<0>(U, gamma, L, R);
+    for (int q = 0; q < 16; ++q) { o[q] = L[q]; o[16 + q] = R[q]; }
+    eigen_matrices<1>(U, gamma, L, R);
+    for (int q = 0; q < 16; ++q) { o[32 + q] = L[q]; o[48 + q] = R[q]; }
+    flux_split<0>(U, gamma, fm, fp);
+    for (int r = 0; r < 4; ++r) { o[64 + r] = fm[r]; o[68 + r] = fp[r]; }
+    flux_split<1>(U, gamma, fm, fp);
+    for (int r = 0; r < 4; ++r) { o[72 + r] = fm[r]; o[76 + r] = fp[r]; }
+    const Prim q = primitive(U[0], U[1], U[2], U[3], gamma);
+    o[80] = fabs(q.u) + q.c;
+    o[81] = fabs(q.v) + q.c;
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers.
+// ---------------------------------------------------------------------------
+hipError_t launch_ghost_fill(double* U, const double* special, const Geom& g, const Bc& bc, hipStream_t s) {
+    const int n = 8 * (g.nx + g.ny);
+    hipLaunchKernelGGL(k_ghost_fill, dim3((n + 255) / 256), dim3(256), 0, s, U, special, g, bc);
+    return hipGetLastError();
+}
+
+template <int S>
+static void launch_fx_s(const double* U, double* Fx, const Geom& g, hipStream_t s) {
+    dim3 grid((g.nx + 1 + kXT - 1) / kXT, g.ny);
+    hipLaunchKernelGGL(k_flux_x<S>, grid, dim3(kXT), 0, s, U, Fx, g);
+}
+
+template <int S>
+static void launch_fy_s(const double* U, double* Fy, const Geom& g, hipStream_t s) {
+    dim3 grid((g.nx + kYC - 1) / kYC, (g.ny + 1 + kYF - 1) / kYF);
+    hipLaunchKernelGGL(k_flux_y<S>, grid, dim3(kYC * kYR), 0, s, U, Fy, g);
+}
+
+hipError_t launch_flux_x(int scheme, const double* U, double* Fx, const Geom& g, hipStream_t s) {
+    switch (scheme) {
+        case kJS: launch_fx_s<kJS>(U, Fx, g, s); break;
+        case kZ: launch_fx_s<kZ>(U, Fx, g, s); break;
+        case kZP: launch_fx_s<kZP>(U, Fx, g, s); break;
+        default: launch_fx_s<kZPI>(U, Fx, g, s); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_flux_y(int scheme, const double* U, double* Fy, const Geom& g, hipStream_t s) {
+    switch (scheme) {
+        case kJS: launch_fy_s<kJS>(U, Fy, g, s); break;
+        case kZ: launch_fy_s<kZ>(U, Fy, g, s); break;
+        case kZP: launch_fy_s<kZP>(U, Fy, g, s); break;
+        default: launch_fy_s<kZPI>(U, Fy, g, s); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rk_update(double* U, double* Un, const double* Fx, const double* Fy, double* rhs_out,
+                            const Geom& g, const RkCoef& k, hipStream_t s) {
+    dim3 grid((g.nx + 255) / 256, g.ny);
+    hipLaunchKernelGGL(k_rk_update, grid, dim3(256), 0, s, U, Un, Fx, Fy, rhs_out, g, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_dt(const double* U, const Geom& g, double cfl, double* part, int* part_nan, int nblocks,
+                     double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_dt_partial, dim3(nblocks), dim3(256), 0, s, U, g, cfl, part, part_nan);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, part, part_nan, nblocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_weno(int scheme, const double* v5, int n, double zp_lambda, double* out, hipStream_t s) {
+    dim3 grid((n + 255) / 256), block(256);
+    switch (scheme) {
+        case kJS: hipLaunchKernelGGL(k_probe_weno<kJS>, grid, block, 0, s, v5, n, zp_lambda, out); break;
+        case kZ: hipLaunchKernelGGL(k_probe_weno<kZ>, grid, block, 0, s, v5, n, zp_lambda, out); break;
+        case kZP: hipLaunchKernelGGL(k_probe_weno<kZP>, grid, block, 0, s, v5, n, zp_lambda, out); break;
+        default: hipLaunchKernelGGL(k_probe_weno<kZPI>, grid, block, 0, s, v5, n, zp_lambda, out); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_physics(const double* U, int n, double gamma, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_probe_physics, dim3((n + 127) / 128), dim3(128), 0, s, U, n, gamma, out);
+    return hipGetLastError();
+}
+
+}  // namespace wenofd

@@ -1,0 +1,154 @@
+/*
+ * wenofd_amd.h — C ABI of the MI355X WENO-FD 2-D Euler solver (libwenofd_amd.so).
+ *
+ * Drop-in for the hot path of infinity-07/FD-WENO-EulerEquation: the CWENOFD
+ * class surface (include/WENOFD.hpp:28-124 in the reference) and its
+ * config.cfg keys (utils/readConfig.hpp:39-80, src/WENOFD.cpp:24-40), with the
+ * MPI slab decomposition replaced by one whole domain on one GPU.
+ *
+ * Conventions
+ *   - Every call returns an int status: WENOFD_OK (0) or a WENOFD_E_* code;
+ *     wenofd_last_error() describes the last failure of the calling thread.
+ *     Nothing in the library aborts the process (the reference calls
+ *     MPI_Abort / exit / throws, src/WENOFD.cpp:234,341,386,441,843).
+ *   - A solver handle owns its device buffers and one HIP stream on one
+ *     device; it is not thread-safe. Distinct handles may be used from
+ *     distinct threads. Host buffers passed in are caller-owned.
+ *   - All solver arithmetic is IEEE FP64 without contraction; results equal a
+ *     no-contraction build of the reference bit for bit.
+ *   - State layouts for get/set: WENOFD_LAYOUT_REF = interior points in the
+ *     reference's Array2D<Cvector> order [NX][NY][4] (x outer, y inner, then
+ *     rho, rho u, rho v, E); WENOFD_LAYOUT_SOA = interior [4][NY][NX];
+ *     WENOFD_LAYOUT_REF_GHOSTED = [(NX+8)][(NY+8)][4] including the 4 ghost
+ *     points per side (corner ghosts are never read by the solver; get
+ *     returns them as zero).
+ */
+#ifndef WENOFD_AMD_H
+#define WENOFD_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wenofd_solver wenofd_solver;
+
+enum {
+    WENOFD_OK = 0,
+    WENOFD_E_CONFIG = 1,   /* missing/unparsable key or unreadable file */
+    WENOFD_E_BAD_ENUM = 2, /* TESTCASE, SCHEME or RKMETH out of range */
+    WENOFD_E_NAN_DT = 3,   /* a NaN time step (non-physical state) */
+    WENOFD_E_HIP = 4,      /* HIP runtime failure or no usable device */
+    WENOFD_E_ARG = 5,      /* invalid argument (null pointer, bad layout, size) */
+    WENOFD_E_IO = 6        /* output file could not be written */
+};
+
+enum { WENOFD_LAYOUT_REF = 0, WENOFD_LAYOUT_SOA = 1, WENOFD_LAYOUT_REF_GHOSTED = 2 };
+
+/* Timing kinds for wenofd_get_timing(). */
+enum {
+    WENOFD_K_GHOST = 0,
+    WENOFD_K_FLUX_X = 1,
+    WENOFD_K_FLUX_Y = 2,
+    WENOFD_K_RK = 3,
+    WENOFD_K_DT = 4,
+    WENOFD_K_COUNT = 5
+};
+
+typedef struct wenofd_info {
+    double cfl, dx, dy, xL, xR, yL, yR, t_end, now, gamma, zp_lambda;
+    long long steps; /* completed time steps (run loop iterations) */
+    int nx, ny, testcase, scheme, rkmeth, device;
+} wenofd_info;
+
+/* Error text of the calling thread's last failed call ("" if none). */
+const char* wenofd_last_error(void);
+/* ABI version (major * 100 + minor). */
+int wenofd_version(void);
+
+/*
+ * CWENOFD::initializeSolver + initializeAve (src/WENOFD.cpp:10-188):
+ * config_text holds "KEY = value" lines as in input/config.cfg ('#' comment
+ * lines, CR-LF tolerated). Required keys TESTCASE (0 SMOOTH, 1 Riemann1,
+ * 2 Riemann2, 3 RTI, 4 isentropic vortex, 5 four-quadrant), SCHEME (0 JS, 1 Z,
+ * 2 ZP, 3 ZPI), RKMETH (0 RK1, 1 RK2, 2 RK3), CFL, ELEMNUMX, ELEMNUMY.
+ * Optional: TEND, VORTEX_EPS/VORTEX_XC/VORTEX_YC, QUAD_STATES (16 numbers),
+ * QUAD_SEED, OUTPUTDIR. device < 0 uses the current device.
+ */
+int wenofd_create(const char* config_text, int device, wenofd_solver** out);
+/* CConfig(path).read(option) then wenofd_create (utils/readConfig.hpp:39-80). */
+int wenofd_create_from_file(const char* path, int device, wenofd_solver** out);
+int wenofd_destroy(wenofd_solver* h);
+
+/* Parses config text without touching a GPU; fills info (nx, ny, enums, cfl,
+ * domain, t_end, gamma, dx, dy). Host-only: no device needed. */
+int wenofd_parse_config(const char* config_text, wenofd_info* info);
+
+/* CWENOFD::initializeAve (src/WENOFD.cpp:165-188): resets the state to the
+ * initial condition and the time to 0. */
+int wenofd_initialize_ave(wenofd_solver* h);
+
+/* CWENOFD::calculateDeltaT (src/WENOFD.cpp:398-452): CFL step, clipped to
+ * t_end - now. WENOFD_E_NAN_DT if any point gives a NaN. */
+int wenofd_calculate_delta_t(wenofd_solver* h, double* dt);
+
+/* CWENOFD::RunRK1/RunRK2/RunRK3 (src/WENOFD.cpp:301-396), rkmeth 0/1/2. Does
+ * not advance the time (run() does). */
+int wenofd_run_rk(wenofd_solver* h, int rkmeth, double dt);
+
+/* exchangeGhostCellsValue + setBoundary (src/WENOFD.cpp:850-929, 998-1159). */
+int wenofd_set_boundary(wenofd_solver* h);
+
+/* getFlux: CWENOFD::assembleRHS = ghost fill + assembleBoundaryTerm +
+ * assembleSourceTerm (src/WENOFD.cpp:454-465, 481-821) on the current state;
+ * the result is read with wenofd_get_rhs. */
+int wenofd_get_flux(wenofd_solver* h);
+int wenofd_get_rhs(wenofd_solver* h, double* host, int layout);
+
+/* One iteration of the run() loop (src/WENOFD.cpp:213-239): dt =
+ * calculateDeltaT (or dt_fixed if > 0), RunRK<RKMETH>(dt), now += dt. */
+int wenofd_step(wenofd_solver* h, double dt_fixed, double* dt_used);
+/* Repeats wenofd_step while |now - t_end| > 1e-9 and fewer than max_steps
+ * (max_steps <= 0: no bound) steps were taken. */
+int wenofd_advance(wenofd_solver* h, int max_steps, double dt_fixed, int* steps_done);
+/* CWENOFD::run (src/WENOFD.cpp:189-299) after creation: outputAve("initial"),
+ * the loop to t_end with progress lines every 20 steps (verbose != 0), then
+ * outputAve("final") and, when an exact solution exists, outputAccuracy and
+ * outputError("final"). Files go to <OUTPUTDIR>/<stamp>_<CASE>_<SCHEME>_<NX>x<NY>/. */
+int wenofd_run(wenofd_solver* h, int verbose);
+
+int wenofd_get_state(wenofd_solver* h, double* host, int layout);
+int wenofd_set_state(wenofd_solver* h, const double* host, int layout);
+int wenofd_get_info(wenofd_solver* h, wenofd_info* info);
+int wenofd_set_time(wenofd_solver* h, double now);
+/* The solve's output directory (created on first output). */
+int wenofd_output_dir(wenofd_solver* h, char* buf, int buflen);
+
+/* outputAve / outputError / outputAccuracy (src/WENOFD.cpp:1161-1382). */
+int wenofd_output_ave(wenofd_solver* h, const char* prefix);
+int wenofd_output_error(wenofd_solver* h, const char* prefix);
+int wenofd_output_accuracy(wenofd_solver* h, const char* prefix, double* norms3);
+/* L_inf, L1, L2 of rho against the exact solution at the current time,
+ * without writing a file (norms3 = {Linf, L1, L2}). */
+int wenofd_accuracy(wenofd_solver* h, double* norms3);
+
+/* Per-kernel timing with HIP events on the solver's stream. While enabled,
+ * each launch is bracketed by events; wenofd_get_timing synchronises and
+ * returns the summed milliseconds and launch counts per WENOFD_K_* kind, then
+ * clears them. */
+int wenofd_set_timing(wenofd_solver* h, int enable);
+int wenofd_get_timing(wenofd_solver* h, double* ms, long long* counts, int nkinds);
+/* Blocks until the solver's stream is idle. */
+int wenofd_synchronize(wenofd_solver* h);
+
+/* Probes of the device arithmetic on independent inputs (host arrays):
+ * WENO5 reconstruction (include/Utility.hpp:415-615) of n 5-tuples, and per
+ * state {Lx, Rx, Ly, Ry (16 each, row-major), Fm_x, Fp_x, Fm_y, Fp_y (4
+ * each), max eigenvalue x, y} = 82 doubles (utils/equations.hpp:137-393). */
+int wenofd_probe_weno5(int device, int scheme, double zp_lambda, int n, const double* v5, double* out);
+int wenofd_probe_physics(int device, double gamma, int n, const double* states, double* out82);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WENOFD_AMD_H */

@@ -1,0 +1,189 @@
+"""ctypes wrapper over oracle/liboracle.so — the CPU oracle.
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker, never as the measured or
+shipped path. See wenofd_oracle.cpp for what it restates and how it is pinned.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+TESTCASES = {"SMOOTH": 0, "RIEMANN1": 1, "RIEMANN2": 2, "RTI": 3, "VORTEX": 4, "QUADRANT": 5}
+SCHEMES = {"JS": 0, "Z": 1, "ZP": 2, "ZPI": 3}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        L = ctypes.CDLL(path)
+        d, i, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+        L.oracle_weno5.restype = d
+        L.oracle_weno5.argtypes = [i, d, d, d, d, d, d]
+        L.oracle_weno5_batch.argtypes = [i, i, _dp, d, _dp]
+        L.oracle_eigen.argtypes = [i, _dp, d, _dp, _dp]
+        L.oracle_split.argtypes = [i, _dp, d, _dp, _dp]
+        L.oracle_phys_flux.argtypes = [i, _dp, d, _dp]
+        L.oracle_max_eigen.restype = d
+        L.oracle_max_eigen.argtypes = [_dp, d, d, d]
+        L.oracle_face_flux.argtypes = [i, i, d, d, _dp, _dp]
+        L.oracle_seeded_quadrant.argtypes = [ctypes.c_ulonglong, _dp]
+        L.oracle_create.restype = vp
+        L.oracle_create.argtypes = [i, i, i, d, i, i, d, _dp, _dp]
+        L.oracle_destroy.argtypes = [vp]
+        L.oracle_grid.argtypes = [vp, _dp, _dp, _dp]
+        for n in ("oracle_get_full", "oracle_set_full", "oracle_get_interior", "oracle_set_interior",
+                  "oracle_assemble_rhs"):
+            getattr(L, n).argtypes = [vp, _dp]
+        L.oracle_fill_ghosts.argtypes = [vp]
+        L.oracle_delta_t.restype = d
+        L.oracle_delta_t.argtypes = [vp]
+        L.oracle_step.restype = i
+        L.oracle_step.argtypes = [vp, i, d]
+        L.oracle_run.restype = i
+        L.oracle_run.argtypes = [vp, i, d]
+        L.oracle_now.restype = d
+        L.oracle_now.argtypes = [vp]
+        L.oracle_accuracy.argtypes = [vp, _dp]
+        L.oracle_count_ops.argtypes = [i, i, i, i, i, ctypes.POINTER(ctypes.c_ulonglong)]
+        _LIB = L
+    return _LIB
+
+
+def weno5(scheme, v5, zp_lambda=0.0):
+    v5 = np.ascontiguousarray(v5, dtype=np.float64).reshape(-1, 5)
+    out = np.empty(v5.shape[0])
+    lib().oracle_weno5_batch(int(scheme), v5.shape[0], _ptr(v5), float(zp_lambda), _ptr(out))
+    return out
+
+
+def eigen(direction, U, gamma=1.4):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    L = np.empty(16)
+    R = np.empty(16)
+    lib().oracle_eigen(int(direction), _ptr(U), gamma, _ptr(L), _ptr(R))
+    return L.reshape(4, 4), R.reshape(4, 4)
+
+
+def split(direction, U, gamma=1.4):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    Fm, Fp = np.empty(4), np.empty(4)
+    lib().oracle_split(int(direction), _ptr(U), gamma, _ptr(Fm), _ptr(Fp))
+    return Fm, Fp
+
+
+def phys_flux(direction, U, gamma=1.4):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    F = np.empty(4)
+    lib().oracle_phys_flux(int(direction), _ptr(U), gamma, _ptr(F))
+    return F
+
+
+def max_eigen(U, nx, ny, gamma=1.4):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    return lib().oracle_max_eigen(_ptr(U), float(nx), float(ny), gamma)
+
+
+def face_flux(direction, scheme, U6, gamma=1.4, zp_lambda=0.0):
+    U6 = np.ascontiguousarray(U6, dtype=np.float64).reshape(6, 4)
+    out = np.empty(4)
+    lib().oracle_face_flux(int(direction), int(scheme), gamma, zp_lambda, _ptr(U6), _ptr(out))
+    return out
+
+
+def seeded_quadrant(seed):
+    out = np.empty(16)
+    lib().oracle_seeded_quadrant(ctypes.c_ulonglong(seed), _ptr(out))
+    return out.reshape(4, 4)
+
+
+def count_ops(testcase, scheme, rk, nx, ny):
+    out = (ctypes.c_ulonglong * 5)()
+    lib().oracle_count_ops(int(testcase), int(scheme), int(rk), int(nx), int(ny), out)
+    return dict(zip(("add", "mul", "div", "sqrt", "cmp"), list(out)))
+
+
+class Solver:
+    """One oracle solve; mirrors the reference CWENOFD loop on a whole domain."""
+
+    def __init__(self, testcase, scheme, rk, cfl, nx, ny, t_end=-1.0, quad=None, vortex=None):
+        self.nx, self.ny = nx, ny
+        tc = TESTCASES[testcase] if isinstance(testcase, str) else int(testcase)
+        sc = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
+        q = None if quad is None else np.ascontiguousarray(quad, dtype=np.float64).reshape(16)
+        v = None if vortex is None else np.ascontiguousarray(vortex, dtype=np.float64).reshape(3)
+        self._q, self._v = q, v
+        self.h = lib().oracle_create(tc, sc, int(rk), float(cfl), nx, ny, float(t_end),
+                                     None if q is None else _ptr(q), None if v is None else _ptr(v))
+        if not self.h:
+            raise ValueError("oracle_create rejected the arguments")
+        self.rk = int(rk)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def grid(self):
+        dxdy = np.empty(3)
+        xc = np.empty(self.nx + 8)
+        yc = np.empty(self.ny + 8)
+        lib().oracle_grid(self.h, _ptr(dxdy), _ptr(xc), _ptr(yc))
+        return dxdy, xc, yc
+
+    def get_state(self):
+        out = np.empty((self.nx, self.ny, 4))
+        lib().oracle_get_interior(self.h, _ptr(out))
+        return out
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.nx, self.ny, 4)
+        lib().oracle_set_interior(self.h, _ptr(U))
+
+    def get_full(self):
+        out = np.empty((self.nx + 8, self.ny + 8, 4))
+        lib().oracle_get_full(self.h, _ptr(out))
+        return out
+
+    def set_full(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.nx + 8, self.ny + 8, 4)
+        lib().oracle_set_full(self.h, _ptr(U))
+
+    def fill_ghosts(self):
+        lib().oracle_fill_ghosts(self.h)
+
+    def assemble_rhs(self):
+        out = np.empty((self.nx, self.ny, 4))
+        lib().oracle_assemble_rhs(self.h, _ptr(out))
+        return out
+
+    def delta_t(self):
+        return lib().oracle_delta_t(self.h)
+
+    def step(self, dt, rk=None):
+        rc = lib().oracle_step(self.h, self.rk if rk is None else int(rk), float(dt))
+        if rc != 0:
+            raise ValueError("bad RK method")
+
+    def run(self, max_steps=0, dt_fixed=0.0):
+        return lib().oracle_run(self.h, int(max_steps), float(dt_fixed))
+
+    def now(self):
+        return lib().oracle_now(self.h)
+
+    def accuracy(self):
+        out = np.empty(3)
+        lib().oracle_accuracy(self.h, _ptr(out))
+        return out  # Linf, L1, L2 of rho
