@@ -1,0 +1,166 @@
+"""Headline benchmark: WENO-Z RK3 cell-updates/s, FP64, 2048^2, one MI355X.
+
+Workload (one step = one iteration of the reference's run() loop,
+src/WENOFD.cpp:213-239): calculateDeltaT, then RunRK3 = 3 x (ghost fill, x-face
+fluxes, y-face fluxes, fused rhs + stage update), on the isentropic vortex on
+[0,10]^2 (periodic), 2048 x 2048 points, WENO-Z, CFL 0.6. The state is resident
+in HBM before the timed region; the only host traffic per step is the 8-byte
+time-step read-back the reference's loop needs.
+
+Prints one JSON line (see the fields below). --gpus > 1 is refused: this
+solver runs one whole domain on one GPU.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "fd-weno-eulerequation_amd")
+sys.path.insert(0, PKG)
+
+METRIC = "WENO-Z RK3 cell-updates/s (FP64, 2048²) + learned-hybrid speedup at equal L1"
+SCHEMES = {"JS": 0, "Z": 1, "ZP": 2, "ZPI": 3}
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector and matrix)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def flop_counts():
+    exe = os.path.join(PKG, "bin", "wenofd_flops")
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    return json.loads(out)
+
+
+def kernel_algorithmic(kind, counts, scheme, nx, ny, rk_first=False):
+    """Algorithmic FLOPs and HBM bytes of one launch (per-unit figures x units)."""
+    sname = {0: "js", 1: "z", 2: "zp", 3: "zpi"}[scheme]
+    if kind == "flux_x":
+        faces, points = ny * (nx + 1), ny * (nx + 6)
+        flops = faces * counts[f"face_x_{sname}"]["flops"] + points * counts["split_x"]["flops"]
+        nbytes = points * 32 + faces * 32
+    elif kind == "flux_y":
+        faces, points = nx * (ny + 1), nx * (ny + 6)
+        flops = faces * counts[f"face_y_{sname}"]["flops"] + points * counts["split_y"]["flops"]
+        nbytes = points * 32 + faces * 32
+    elif kind == "rk_update":
+        flops = nx * ny * 4 * counts["cell_var_stage"]["flops"]
+        nbytes = nx * ny * (32 + 32 + 32 + 32 + 32)  # Fx, Fy, U, Un (or Un written), U written
+    elif kind == "delta_t":
+        flops = nx * ny * counts["cell_dt"]["flops"]
+        nbytes = nx * ny * 32
+    else:  # ghost_fill
+        flops = 0
+        nbytes = 8 * (nx + ny) * 64
+    return flops, nbytes
+
+
+def cpu_baseline(nx, ny, scheme, cfl, steps):
+    """The CPU oracle (a from-scratch serial restatement of the reference's
+    arithmetic) timed on this host over a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    s = O.Solver(4, scheme, 2, cfl, nx, ny, t_end=100.0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(s.delta_t())
+    el = time.perf_counter() - t0
+    return {"value": nx * ny * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps (calculateDeltaT + RunRK3) of the same {nx}x{ny} vortex, WENO-"
+                      f"{[k for k, v in SCHEMES.items() if v == scheme][0]}, serial C++ oracle, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--scheme", default="Z", choices=list(SCHEMES))
+    ap.add_argument("--cfl", type=float, default=0.6)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for cpu_baseline (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this solver runs one whole domain on one GPU; --gpus must be 1", file=sys.stderr)
+        return 2
+
+    import wenofd as W
+
+    n, sc = args.size, SCHEMES[args.scheme]
+    log(f"vortex {n}x{n} WENO-{args.scheme} RK3 CFL {args.cfl}: create")
+    s = W.Solver(TESTCASE=4, SCHEME=sc, RKMETH=2, CFL=args.cfl, ELEMNUMX=n, ELEMNUMY=n, TEND=1e9)
+    log(f"warmup {args.warmup} steps")
+    s.advance(args.warmup)
+    s.synchronize()
+    s.set_timing(True)
+    log(f"timed {args.steps} steps")
+    t0 = time.perf_counter()
+    done = s.advance(args.steps)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    assert done == args.steps
+    kt = s.get_timing()
+    s.set_timing(False)
+    log(f"{el * 1e3 / args.steps:.3f} ms/step; kernels: " +
+        ", ".join(f"{k} {v[0] / max(v[1], 1) * 1e3:.1f} us x{v[1]}" for k, v in kt.items()))
+
+    counts = flop_counts()
+    kernels = {}
+    for k, (ms, cnt) in kt.items():
+        if cnt == 0:
+            continue
+        flops, nbytes = kernel_algorithmic(k, counts, sc, n, n)
+        avg_s = ms / cnt / 1e3
+        kernels[k] = {"avg_us": avg_s * 1e6, "launches": cnt, "flops_per_launch": flops,
+                      "bytes_per_launch": nbytes, "tflops": flops / avg_s / 1e12, "gbs": nbytes / avg_s / 1e9}
+    dom = max(kernels, key=lambda k: kt[k][0])
+    dk = kernels[dom]
+    value = n * n * args.steps / el
+    step_flops = sum(kernels[k]["flops_per_launch"] * kernels[k]["launches"] for k in kernels) / args.steps
+    rec = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (isentropic vortex initial condition, generated on the host)",
+        "config": {"workload": f"isentropic vortex {n}x{n}, WENO-{args.scheme}, RK3, CFL {args.cfl}, "
+                               "dt from calculateDeltaT every step",
+                   "nx": n, "ny": n, "scheme": args.scheme, "rk": "RK3", "cfl": args.cfl},
+        "roofline": {
+            "kernel": dom,
+            "bound": "fp64",
+            "achieved": dk["tflops"],
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": dk["tflops"] / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "hbm": {"achieved": dk["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dk["gbs"] / HBM_PEAK_GBS},
+            "step_fp64_tflops": step_flops / (el / args.steps) / 1e12,
+            "kernels": kernels,
+        },
+    }
+    if args.cpu_steps > 0:
+        log(f"cpu baseline: {args.cpu_steps} oracle steps at {n}x{n}")
+        rec["cpu_baseline"] = cpu_baseline(n, n, sc, args.cfl, args.cpu_steps)
+    else:
+        rec["cpu_baseline"] = None
+    print(json.dumps(rec), flush=True)
+    s.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -83,41 +83,6 @@ __global__ __launch_bounds__(256) void k_ghost_fill(double* __restrict__ U, cons
 }
 
 // ---------------------------------------------------------------------------
-// Characteristic-wise LLF-split WENO5 flux at one face (src/WENOFD.cpp:526-644):
-// arithmetic-mean state -> L, R -> project the 5-point F+ (points f-3..f+1) and
-// F- (points f-2..f+2) stencils -> 2 x 4 WENO5 -> R (v+ + v-).
-// sp/sm(s, r) return F+/F- component r of stencil slot s (slot 0 = point f-3).
-// ---------------------------------------------------------------------------
-template <int D, int S, class GetP, class GetM>
-__device__ __forceinline__ void face_flux(const double Ul[4], const double Ur[4], GetP sp, GetM sm, double gamma,
-                                          double zp_lambda, double Fh[4]) {
-    double ave[4], L[16], R[16];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) ave[r] = 0.5 * (Ul[r] + Ur[r]);
-    eigen_matrices<D>(ave, gamma, L, R);
-    double vp[5][4], vm[5][4];
-#pragma unroll
-    for (int s = 0; s < 5; ++s) {
-        double fp[4], fm[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            fp[r] = sp(s, r);
-            fm[r] = sm(s + 1, r);
-        }
-        project_left<D>(L, fp, vp[s]);
-        project_left<D>(L, fm, vm[s]);
-    }
-    double vh[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const double hp = weno5<S>(vp[0][k], vp[1][k], vp[2][k], vp[3][k], vp[4][k], zp_lambda);
-        const double hm = weno5<S>(vm[4][k], vm[3][k], vm[2][k], vm[1][k], vm[0][k], zp_lambda);
-        vh[k] = hp + hm;
-    }
-    project_right<D>(R, vh, Fh);
-}
-
-// ---------------------------------------------------------------------------
 // x faces. One block = 256 consecutive faces of one interior row; the LLF split
 // of the 261 points they read is computed once into LDS (component-major, so a
 // wave's 64 consecutive slots hit 64 distinct banks), then each thread builds
@@ -252,9 +217,8 @@ __global__ __launch_bounds__(256) void k_rk_update(double* __restrict__ U, doubl
     double rhs[4], u[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const double tx = -(Fx[r * g.fx_plane + ox + 1] - Fx[r * g.fx_plane + ox]) / g.dx;
-        const double ty = -(Fy[r * g.fy_plane + oy + g.fy_pitch] - Fy[r * g.fy_plane + oy]) / g.dy;
-        rhs[r] = tx + ty;
+        rhs[r] = cell_rhs(Fx[r * g.fx_plane + ox], Fx[r * g.fx_plane + ox + 1], Fy[r * g.fy_plane + oy],
+                          Fy[r * g.fy_plane + oy + g.fy_pitch], g.dx, g.dy);
         u[r] = U[r * g.plane + o];
     }
     if (k.gravity) {
@@ -279,7 +243,7 @@ __global__ __launch_bounds__(256) void k_rk_update(double* __restrict__ U, doubl
             } else {
                 un = Un[r * g.plane + o];
             }
-            nu = (k.a * un + k.b * u[r]) + k.cdt * rhs[r];
+            nu = rk_combine(k.a, un, k.b, u[r], k.cdt, rhs[r]);
         }
         U[r * g.plane + o] = nu;
     }
@@ -324,10 +288,8 @@ __global__ __launch_bounds__(256) void k_dt_partial(const double* __restrict__ U
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (long)gridDim.x * blockDim.x) {
         const int jl = (int)(k / g.nx), il = (int)(k % g.nx);
         const size_t o = (size_t)(jl + 4) * g.pitch + (il + 4);
-        const Prim q = primitive(U[o], U[g.plane + o], U[2 * g.plane + o], U[3 * g.plane + o], g.gamma);
-        const double ex = fabs(q.u) + q.c;
-        const double ey = fabs(q.v) + q.c;
-        const double tmp = cfl / (ex / g.dx + ey / g.dy);
+        const double u[4] = {U[o], U[g.plane + o], U[2 * g.plane + o], U[3 * g.plane + o]};
+        const double tmp = local_dt(u, g.gamma, cfl, g.dx, g.dy);
         if (tmp != tmp) nan = 1;
         m = (tmp < m) ? tmp : m;
     }
@@ -383,7 +345,7 @@ __global__ void k_probe_physics(const double* __restrict__ Uin, int n, double ga
     for (int r = 0; r < 4; ++r) { o[64 + r] = fm[r]; o[68 + r] = fp[r]; }
     flux_split<1>(U, gamma, fm, fp);
     for (int r = 0; r < 4; ++r) { o[72 + r] = fm[r]; o[76 + r] = fp[r]; }
-    const Prim q = primitive(U[0], U[1], U[2], U[3], gamma);
+    const Prim<double> q = primitive(U[0], U[1], U[2], U[3], gamma);
     o[80] = fabs(q.u) + q.c;
     o[81] = fabs(q.v) + q.c;
 }

@@ -14,6 +14,11 @@
 
 namespace wenofd {
 
+// Every function is a template on the scalar type T: the kernels instantiate
+// T = double; a host tool instantiates an operation-counting type to derive
+// the algorithmic FLOPs per face, per point and per cell from this same code.
+#define WENOFD_HD __host__ __device__ __forceinline__
+
 enum Scheme { kJS = 0, kZ = 1, kZP = 2, kZPI = 3 };
 
 // include/Utility.hpp:5-11
@@ -26,10 +31,10 @@ constexpr double kC13_12 = 13.0 / 12.0;
 
 // Smoothness indicators beta_0..2 of the three 3-point sub-stencils of
 // (a,b,c,d,e) = (f_{j-2},...,f_{j+2}); include/Utility.hpp:427-437.
-__device__ __forceinline__ void weno_beta(double a, double b, double c, double d, double e, double& b0, double& b1,
-                                          double& b2) {
-    double t1 = (a - 2.0 * b) + c;
-    double t2 = (a - 4.0 * b) + 3.0 * c;
+template <class T>
+WENOFD_HD void weno_beta(T a, T b, T c, T d, T e, T& b0, T& b1, T& b2) {
+    T t1 = (a - 2.0 * b) + c;
+    T t2 = (a - 4.0 * b) + 3.0 * c;
     b0 = (kC13_12 * t1) * t1 + (0.25 * t2) * t2;
     t1 = (b - 2.0 * c) + d;
     t2 = b - d;
@@ -40,8 +45,8 @@ __device__ __forceinline__ void weno_beta(double a, double b, double c, double d
 }
 
 // Third-order candidate values at j+1/2; include/Utility.hpp:451-453.
-__device__ __forceinline__ void weno_candidates(double a, double b, double c, double d, double e, double& p0,
-                                                double& p1, double& p2) {
+template <class T>
+WENOFD_HD void weno_candidates(T a, T b, T c, T d, T e, T& p0, T& p1, T& p2) {
     p0 = (kC1_3 * a - kC7_6 * b) + kC11_6 * c;
     p1 = (-kC1_6 * b + kC5_6 * c) + kC1_3 * d;
     p2 = (kC1_3 * c + kC5_6 * d) - kC1_6 * e;
@@ -49,9 +54,8 @@ __device__ __forceinline__ void weno_candidates(double a, double b, double c, do
 
 // Unnormalised nonlinear weights of one scheme (include/Utility.hpp:439-442,
 // 477-491, 530-542, 584-599).
-template <int S>
-__device__ __forceinline__ void weno_alpha(double b0, double b1, double b2, double zp_lambda, double& w0, double& w1,
-                                           double& w2) {
+template <int S, class T>
+WENOFD_HD void weno_alpha(T b0, T b1, T b2, double zp_lambda, T& w0, T& w1, T& w2) {
     if constexpr (S == kJS) {
         const double eps = 1.0e-6;
         w0 = 0.1 / ((eps + b0) * (eps + b0));
@@ -59,34 +63,35 @@ __device__ __forceinline__ void weno_alpha(double b0, double b1, double b2, doub
         w2 = 0.3 / ((eps + b2) * (eps + b2));
     } else if constexpr (S == kZ) {
         const double eps = 1.0e-20;
-        const double i0 = 1.0 / (b0 + eps);
-        const double i1 = 1.0 / (b1 + eps);
-        const double i2 = 1.0 / (b2 + eps);
-        const double tau = fabs(b0 - b2);
+        const T i0 = 1.0 / (b0 + eps);
+        const T i1 = 1.0 / (b1 + eps);
+        const T i2 = 1.0 / (b2 + eps);
+        const T tau = fabs(b0 - b2);
         w0 = 0.1 * (1.0 + (((tau * i0) * tau) * i0));
         w1 = 0.6 * (1.0 + (((tau * i1) * tau) * i1));
         w2 = 0.3 * (1.0 + (((tau * i2) * tau) * i2));
     } else if constexpr (S == kZP) {
         const double eps = 1.0e-40;
-        const double tau = fabs(b0 - b2);
-        const double te = tau + eps;
-        const double q0 = te / (b0 + eps);
-        const double q1 = te / (b1 + eps);
-        const double q2 = te / (b2 + eps);
+        const T tau = fabs(b0 - b2);
+        const T te = tau + eps;
+        const T q0 = te / (b0 + eps);
+        const T q1 = te / (b1 + eps);
+        const T q2 = te / (b2 + eps);
         w0 = 0.1 * ((1.0 + q0 * q0) + (zp_lambda * (b0 + eps)) / te);
         w1 = 0.6 * ((1.0 + q1 * q1) + (zp_lambda * (b1 + eps)) / te);
         w2 = 0.3 * ((1.0 + q2 * q2) + (zp_lambda * (b2 + eps)) / te);
     } else {
         const double eps = 1.0e-40;
-        const double tau = fabs(b0 - b2);
-        const double bmax = (b0 < ((b1 < b2) ? b2 : b1)) ? ((b1 < b2) ? b2 : b1) : b0;  // std::max(b0, std::max(b1, b2))
-        const double bmn12 = (b2 < b1) ? b2 : b1;
-        const double bmin = (bmn12 < b0) ? bmn12 : b0;  // std::min(b0, std::min(b1, b2))
-        const double alpha = 1.0 - (bmin / (bmax + 1e-40));
-        const double la = 3.3 * alpha;
-        const double q0 = tau / (b0 + eps);
-        const double q1 = tau / (b1 + eps);
-        const double q2 = tau / (b2 + eps);
+        const T tau = fabs(b0 - b2);
+        const T bmx12 = (b1 < b2) ? b2 : b1;
+        const T bmax = (b0 < bmx12) ? bmx12 : b0;  // std::max(b0, std::max(b1, b2))
+        const T bmn12 = (b2 < b1) ? b2 : b1;
+        const T bmin = (bmn12 < b0) ? bmn12 : b0;  // std::min(b0, std::min(b1, b2))
+        const T alpha = 1.0 - (bmin / (bmax + 1e-40));
+        const T la = 3.3 * alpha;
+        const T q0 = tau / (b0 + eps);
+        const T q1 = tau / (b1 + eps);
+        const T q2 = tau / (b2 + eps);
         w0 = 0.1 * ((1.0 + q0 * q0) + (la * b0) / (bmax + eps));
         w1 = 0.6 * ((1.0 + q1 * q1) + (la * b1) / (bmax + eps));
         w2 = 0.3 * ((1.0 + q2 * q2) + (la * b2) / (bmax + eps));
@@ -94,12 +99,12 @@ __device__ __forceinline__ void weno_alpha(double b0, double b1, double b2, doub
 }
 
 // Fifth-order WENO reconstruction at j+1/2 from (f_{j-2},...,f_{j+2}).
-template <int S>
-__device__ __forceinline__ double weno5(double a, double b, double c, double d, double e, double zp_lambda) {
-    double b0, b1, b2, w0, w1, w2, p0, p1, p2;
+template <int S, class T>
+WENOFD_HD T weno5(T a, T b, T c, T d, T e, double zp_lambda) {
+    T b0, b1, b2, w0, w1, w2, p0, p1, p2;
     weno_beta(a, b, c, d, e, b0, b1, b2);
     weno_alpha<S>(b0, b1, b2, zp_lambda, w0, w1, w2);
-    const double s = (w0 + w1) + w2;
+    const T s = (w0 + w1) + w2;
     w0 = w0 / s;
     w1 = w1 / s;
     w2 = w2 / s;
@@ -109,12 +114,14 @@ __device__ __forceinline__ double weno5(double a, double b, double c, double d, 
 
 // Primitive variables with the reference's expression forms
 // (utils/equations.hpp:139-144): p = (g-1)(E - (0.5 rho)(u^2+v^2)), c = sqrt((g p)/rho).
+template <class T>
 struct Prim {
-    double rho, u, v, E, p, c;
+    T rho, u, v, E, p, c;
 };
 
-__device__ __forceinline__ Prim primitive(double U0, double U1, double U2, double U3, double gamma) {
-    Prim q;
+template <class T>
+WENOFD_HD Prim<T> primitive(T U0, T U1, T U2, T U3, double gamma) {
+    Prim<T> q;
     q.rho = U0;
     q.u = U1 / U0;
     q.v = U2 / U0;
@@ -127,24 +134,24 @@ __device__ __forceinline__ Prim primitive(double U0, double U1, double U2, doubl
 // Local Lax-Friedrichs split at one point along direction D (0 = x, 1 = y):
 // F-+ = 0.5 (F -+ alpha U), alpha = |u_n| + c at this point.
 // utils/equations.hpp:105-134 (physical flux), :365-393 (split).
-template <int D>
-__device__ __forceinline__ void flux_split(const double U[4], double gamma, double Fm[4], double Fp[4]) {
-    const Prim q = primitive(U[0], U[1], U[2], U[3], gamma);
-    double F[4];
+template <int D, class T>
+WENOFD_HD void flux_split(const T U[4], double gamma, T Fm[4], T Fp[4]) {
+    const Prim<T> q = primitive(U[0], U[1], U[2], U[3], gamma);
+    T F[4];
     if constexpr (D == 0) {
-        const double ru = q.rho * q.u;
+        const T ru = q.rho * q.u;
         F[0] = ru;
         F[1] = ru * q.u + q.p;
         F[2] = ru * q.v;
         F[3] = (q.E + q.p) * q.u;
     } else {
-        const double rv = q.rho * q.v;
+        const T rv = q.rho * q.v;
         F[0] = rv;
         F[1] = (q.rho * q.u) * q.v;
         F[2] = rv * q.v + q.p;
         F[3] = (q.E + q.p) * q.v;
     }
-    const double ws = (D == 0 ? fabs(q.u) : fabs(q.v)) + q.c;
+    const T ws = (D == 0 ? fabs(q.u) : fabs(q.v)) + q.c;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         Fm[r] = 0.5 * (F[r] - ws * U[r]);
@@ -155,17 +162,17 @@ __device__ __forceinline__ void flux_split(const double U[4], double gamma, doub
 // Left (rows = left eigenvectors) and right eigenmatrices of dF/dU along D at
 // state U, with the reference's normalisation (utils/equations.hpp:174-299).
 // Row-major 4x4: L[i*4+j].
-template <int D>
-__device__ __forceinline__ void eigen_matrices(const double U[4], double gamma, double L[16], double R[16]) {
-    const Prim q = primitive(U[0], U[1], U[2], U[3], gamma);
-    const double B1 = (gamma - 1.0) / (q.c * q.c);
-    const double uu = q.u * q.u + q.v * q.v;
-    const double B2 = (B1 * uu) / 2.0;
-    const double ic = 1.0 / q.c;
-    const double H = (q.E + q.p) / q.rho;
-    const double B1u = B1 * q.u, B1v = B1 * q.v;
+template <int D, class T>
+WENOFD_HD void eigen_matrices(const T U[4], double gamma, T L[16], T R[16]) {
+    const Prim<T> q = primitive(U[0], U[1], U[2], U[3], gamma);
+    const T B1 = (gamma - 1.0) / (q.c * q.c);
+    const T uu = q.u * q.u + q.v * q.v;
+    const T B2 = (B1 * uu) / 2.0;
+    const T ic = 1.0 / q.c;
+    const T H = (q.E + q.p) / q.rho;
+    const T B1u = B1 * q.u, B1v = B1 * q.v;
     if constexpr (D == 0) {
-        const double uc = q.u / q.c;
+        const T uc = q.u / q.c;
         L[0] = (B2 + uc) / 2.0;  L[1] = -(B1u + ic) / 2.0;  L[2] = -B1v / 2.0;  L[3] = B1 / 2.0;
         L[4] = q.v;              L[5] = 0.0;                L[6] = -1.0;        L[7] = 0.0;
         L[8] = 1.0 - B2;         L[9] = B1u;                L[10] = B1v;        L[11] = -B1;
@@ -175,7 +182,7 @@ __device__ __forceinline__ void eigen_matrices(const double U[4], double gamma, 
         R[8] = q.v;              R[9] = -1.0;  R[10] = q.v;       R[11] = q.v;
         R[12] = H - q.c * q.u;   R[13] = -q.v; R[14] = uu / 2.0;  R[15] = H + q.c * q.u;
     } else {
-        const double vc = q.v / q.c;
+        const T vc = q.v / q.c;
         L[0] = (B2 + vc) / 2.0;  L[1] = -B1u / 2.0; L[2] = -(B1v + ic) / 2.0;  L[3] = B1 / 2.0;
         L[4] = -q.u;             L[5] = 1.0;        L[6] = 0.0;                L[7] = 0.0;
         L[8] = 1.0 - B2;         L[9] = B1u;        L[10] = B1v;               L[11] = -B1;
@@ -190,8 +197,8 @@ __device__ __forceinline__ void eigen_matrices(const double U[4], double gamma, 
 // Projection of one split-flux vector onto the characteristic fields:
 // v_i = ((L_i0 f0 + L_i1 f1) + L_i2 f2) + L_i3 f3 (src/WENOFD.cpp:552-593),
 // with the exact-zero / unit coefficients of row 1 folded.
-template <int D>
-__device__ __forceinline__ void project_left(const double L[16], const double f[4], double v[4]) {
+template <int D, class T>
+WENOFD_HD void project_left(const T L[16], const T f[4], T v[4]) {
     v[0] = ((L[0] * f[0] + L[1] * f[1]) + L[2] * f[2]) + L[3] * f[3];
     if constexpr (D == 0)
         v[1] = L[4] * f[0] - f[2];  // row (v, 0, -1, 0)
@@ -202,8 +209,8 @@ __device__ __forceinline__ void project_left(const double L[16], const double f[
 }
 
 // Back-projection F = R vh (src/WENOFD.cpp:625-643), unit/zero coefficients folded.
-template <int D>
-__device__ __forceinline__ void project_right(const double R[16], const double vh[4], double F[4]) {
+template <int D, class T>
+WENOFD_HD void project_right(const T R[16], const T vh[4], T F[4]) {
     F[0] = (vh[0] + vh[2]) + vh[3];  // row (1, 0, 1, 1)
     if constexpr (D == 0) {
         F[1] = (R[4] * vh[0] + R[6] * vh[2]) + R[7] * vh[3];        // (u-c, 0, u, u+c)
@@ -213,6 +220,64 @@ __device__ __forceinline__ void project_right(const double R[16], const double v
         F[2] = (R[8] * vh[0] + R[10] * vh[2]) + R[11] * vh[3];       // (v-c, 0, v, v+c)
     }
     F[3] = ((R[12] * vh[0] + R[13] * vh[1]) + R[14] * vh[2]) + R[15] * vh[3];
+}
+
+// Characteristic-wise LLF-split WENO5 flux at one face (src/WENOFD.cpp:526-644):
+// arithmetic-mean state -> L, R -> project the 5-point F+ (points f-3..f+1) and
+// F- (points f-2..f+2) stencils -> 2 x 4 WENO5 -> R (v+ + v-).
+// sp(s, r) / sm(s, r) return component r of F+ / F- at stencil slot s
+// (slot 0 = point f-3, slot 5 = point f+2).
+template <int D, int S, class T, class GetP, class GetM>
+WENOFD_HD void face_flux(const T Ul[4], const T Ur[4], GetP sp, GetM sm, double gamma, double zp_lambda, T Fh[4]) {
+    T ave[4], L[16], R[16];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ave[r] = 0.5 * (Ul[r] + Ur[r]);
+    eigen_matrices<D>(ave, gamma, L, R);
+    T vp[5][4], vm[5][4];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        T fp[4], fm[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            fp[r] = sp(s, r);
+            fm[r] = sm(s + 1, r);
+        }
+        project_left<D>(L, fp, vp[s]);
+        project_left<D>(L, fm, vm[s]);
+    }
+    T vh[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const T hp = weno5<S>(vp[0][k], vp[1][k], vp[2][k], vp[3][k], vp[4][k], zp_lambda);
+        const T hm = weno5<S>(vm[4][k], vm[3][k], vm[2][k], vm[1][k], vm[0][k], zp_lambda);  // mirrored (:604-605)
+        vh[k] = hp + hm;
+    }
+    project_right<D>(R, vh, Fh);
+}
+
+// Local CFL step of one point (src/WENOFD.cpp:419-423, utils/equations.hpp:137-147):
+// cfl / ((|u| + c)/dx + (|v| + c)/dy); |u*1 + v*0| == |u| for every finite v.
+template <class T>
+WENOFD_HD T local_dt(const T U[4], double gamma, double cfl, double dx, double dy) {
+    const Prim<T> q = primitive(U[0], U[1], U[2], U[3], gamma);
+    const T ex = fabs(q.u) + q.c;
+    const T ey = fabs(q.v) + q.c;
+    return cfl / (ex / dx + ey / dy);
+}
+
+// rhs of one conserved variable at one cell: -(Fx[i+1]-Fx[i])/dx - (Fy[j+1]-Fy[j])/dy
+// (src/WENOFD.cpp:646-648, 794-796; the x term is added to a zeroed rhs first).
+template <class T>
+WENOFD_HD T cell_rhs(T fxl, T fxr, T fyb, T fyt, double dx, double dy) {
+    const T tx = -(fxr - fxl) / dx;
+    const T ty = -(fyt - fyb) / dy;
+    return tx + ty;
+}
+
+// RK stage combination (src/WENOFD.cpp:394): (a Un + b U) + (c dt) rhs.
+template <class T>
+WENOFD_HD T rk_combine(double a, T un, double b, T u, double cdt, T rhs) {
+    return (a * un + b * u) + cdt * rhs;
 }
 
 }  // namespace wenofd

@@ -1,0 +1,243 @@
+"""GPU parity through the C ABI (libwenofd_amd.so) against the golden vectors
+and the CPU oracle.
+
+Bar: bit-for-bit equality (IEEE ==, so +0 == -0) with the reference's
+no-contraction build and the oracle; against the reference's makefile build
+(FMA-contracted) the relative max-norm per conserved variable stays <= 1e-11
+(RTI excepted: its rhs is ill-conditioned, see test_oracle_golden.py).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import wenofd as W
+from conftest import relmax, same
+
+pytestmark = pytest.mark.gpu
+
+CASE_KEYS = {0: "SMOOTH", 1: "RIEMANN1", 2: "RIEMANN2", 3: "RTI", 4: "VORTEX", 5: "QUADRANT"}
+
+
+def solver(tc, sc, rk, cfl, nx, ny, **extra):
+    print(f"[gpu] solver tc={tc} scheme={sc} rk={rk} cfl={cfl} {nx}x{ny} {extra}", flush=True)
+    keys = dict(TESTCASE=tc, SCHEME=sc, RKMETH=rk, CFL="%.17g" % cfl, ELEMNUMX=nx, ELEMNUMY=ny)
+    keys.update(extra)
+    return W.Solver(**keys)
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
+def test_probe_weno_bitwise(golden, scheme):
+    g = golden("weno_vectors")
+    lam = math.pow(float(g["deltaX"]), 2.0 / 3.0)
+    out = W.probe_weno5(scheme, g["inputs"], lam)
+    assert same(out, g[f"ref_nofma_s{scheme}"])
+    assert same(out, O.weno5(scheme, g["inputs"], lam))
+
+
+@pytest.mark.parametrize("tag", ["g14", "g53"])
+def test_probe_physics_bitwise(golden, tag):
+    g = golden("physics_vectors")
+    out = W.probe_physics(g[f"{tag}_states"], float(g[f"{tag}_gamma"]))
+    assert same(out, g[f"{tag}_ref_nofma"][:, :82])
+
+
+@pytest.mark.parametrize("tc", [0, 1, 2, 3])
+@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
+def test_get_flux_ghosts_dt_bitwise(golden, tc, scheme):
+    g = golden("rhs_cases")
+    U = g[f"tc{tc}_state"]
+    nx, ny = U.shape[:2]
+    with solver(tc, scheme, 2, 0.6, nx, ny) as s:
+        s.set_state(U)
+        s.set_boundary()
+        G = s.get_state(W.LAYOUT_REF_GHOSTED)
+        ref = g[f"tc{tc}_ghosted"]
+        m = np.ones((nx + 8, ny + 8), bool)
+        m[:4, :4] = m[:4, -4:] = m[-4:, :4] = m[-4:, -4:] = False  # corner ghosts are never read
+        assert same(G[m], ref[m])
+        s.get_flux()
+        rhs = s.get_rhs()
+        assert same(rhs, g[f"tc{tc}_s{scheme}_ref_nofma_rhs"])
+        if tc != 3:
+            assert relmax(rhs, g[f"tc{tc}_s{scheme}_ref_native_rhs"]) < 1e-12
+        assert s.calculate_delta_t() == float(g[f"tc{tc}_s{scheme}_ref_nofma_dt"])
+
+
+def _traj_names(g):
+    return sorted({k[: -len("_spec")] for k in g.files if k.endswith("_spec")})
+
+
+def test_trajectories_bitwise(golden):
+    g = golden("trajectories")
+    for name in _traj_names(g):
+        tc, sc, rk, cfl, nx, ny, ns, fx, te = g[f"{name}_spec"]
+        tc, sc, rk, nx, ny, ns, fx = map(int, (tc, sc, rk, nx, ny, ns, fx))
+        extra = g[f"{name}_extra"]
+        keys = {}
+        if te >= 0:
+            keys["TEND"] = "%.17g" % te
+        if tc == 4 and extra.size == 3:
+            keys.update(VORTEX_EPS="%.17g" % extra[0], VORTEX_XC="%.17g" % extra[1], VORTEX_YC="%.17g" % extra[2])
+        if tc == 5 and extra.size == 16:
+            keys["QUAD_STATES"] = ",".join("%.17g" % v for v in extra)
+        with solver(tc, sc, rk, cfl, nx, ny, **keys) as s:
+            dts = []
+            if ns < 0:
+                while abs(s.info().now - te) > 1e-9:
+                    dts.append(s.step())
+            else:
+                dt0 = 0.0
+                for k in range(ns):
+                    dts.append(s.step(dt0 if fx else 0.0))
+                    if k == 0:
+                        dt0 = dts[0]
+            U = s.get_state()
+        assert same(np.array(dts), g[f"{name}_ref_nofma_dts"]), name
+        assert same(U, g[f"{name}_ref_nofma_state"]), name
+        if not name.startswith("rti"):
+            assert relmax(U, g[f"{name}_ref_native_state"]) < 1e-11, name
+
+
+def test_run_rk_methods_match_oracle():
+    """RunRK1/RunRK2/RunRK3 through wenofd_run_rk with an explicit dt."""
+    for rk in (0, 1, 2):
+        with solver(2, 1, rk, 0.5, 20, 18) as s:
+            o = O.Solver(2, 1, rk, 0.5, 20, 18)
+            for _ in range(3):
+                dt = s.calculate_delta_t()
+                assert dt == o.delta_t()
+                s.run_rk(rk, dt)
+                o.step(dt, rk)
+                s.set_time(o.now())
+            assert same(s.get_state(), o.get_state()), rk
+
+
+def test_c2_quadrant_800_200_steps(golden):
+    """Config C2: 800^2 four-quadrant, WENO-Z, RK3, step-0 CFL-0.7 dt, 200 steps."""
+    g = golden("c2_quadrant_800")
+    with solver(5, 1, 2, 0.7, 800, 800) as s:
+        dt0 = s.step()
+        assert dt0 == float(g["ref_nofma_dt0"])
+        for _ in range(199):
+            s.step(dt0)
+        U = s.get_state()
+    idx = g["sample_index"]
+    samp = U[np.ix_(idx, idx)]
+    assert same(samp, g["ref_nofma_sample"])
+    # whole-field checksum: the same numpy summation over a bitwise-equal field
+    assert same(U.sum(axis=(0, 1)), g["ref_nofma_sum"])
+    assert relmax(samp, g["ref_native_sample"]) < 1e-11
+    assert np.all(np.abs(U.sum(axis=(0, 1)) - g["ref_native_sum"]) <= 1e-11 * np.abs(g["ref_native_sum"]))
+
+
+def test_vortex_100_to_t2(golden):
+    """Config C1: isentropic vortex 100^2, WENO-JS, RK3, CFL 0.05, t_end = 2."""
+    g = golden("vortex_100")
+    with solver(4, 0, 2, 0.05, 100, 100) as s:
+        n = s.advance()
+        U = s.get_state()
+        acc = s.accuracy()
+    assert n == int(g["ref_nofma_nsteps"])
+    assert same(U, g["ref_nofma_state"])
+    print(f"[gpu] vortex 100^2 JS t=2: Linf={acc[0]:.3e} L1={acc[1]:.3e}")
+    assert acc[0] < 1e-1 and acc[1] < 1e-2
+
+
+# README.md:64-74 (also docs/code_introduction.tex:424-440): SMOOTH, CFL 0.05,
+# RK3, t = 1; L_inf and L1 of rho at N = 100..200, three significant digits.
+KAT = {
+    0: {100: (6.42e-07, 1.92e-07), 120: (2.56e-07, 7.58e-08), 140: (1.17e-07, 3.45e-08),
+        160: (5.81e-08, 1.74e-08), 180: (3.12e-08, 9.53e-09), 200: (1.78e-08, 5.55e-09)},
+    1: {100: (1.66e-07, 4.21e-08), 120: (6.64e-08, 1.68e-08), 140: (3.06e-08, 7.73e-09),
+        160: (1.56e-08, 3.95e-09), 180: (8.65e-09, 2.19e-09), 200: (5.10e-09, 1.29e-09)},
+}
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_smooth_convergence_table(golden, scheme):
+    res = {}
+    for n in sorted(KAT[scheme]):
+        with solver(0, scheme, 2, 0.05, n, n) as s:
+            s.advance()
+            assert abs(s.info().now - 1.0) <= 1e-9
+            res[n] = s.accuracy()
+        for got, printed in zip(res[n][:2], KAT[scheme][n]):
+            # three significant digits: within half a unit of the last printed digit
+            half_unit = 0.5 * 10.0 ** (math.floor(math.log10(printed)) - 2)
+            assert abs(got - printed) <= 1.001 * half_unit, (n, got, printed)
+    ns = sorted(res)
+    for a, b in zip(ns[:-1], ns[1:]):
+        order = math.log(res[a][1] / res[b][1]) / math.log(b / a)
+        assert 4.8 < order < 5.4, (a, b, order)
+    ref = golden("run_accuracy")[f"smooth_s{scheme}_n100_cfl0.05"]
+    assert np.allclose(res[100], ref[1:], rtol=1e-9, atol=0)
+
+
+def test_seeded_quadrant_initial_state():
+    with solver(5, 1, 2, 0.6, 33, 31, QUAD_SEED=3) as s:
+        U = s.get_state()
+    o = O.Solver(5, 1, 2, 0.6, 33, 31, quad=O.seeded_quadrant(3))
+    assert same(U, o.get_state())
+
+
+def test_nan_time_step_is_reported():
+    with solver(2, 1, 2, 0.6, 24, 24) as s:
+        U = s.get_state()
+        U[5, 7, 3] = -1.0  # negative energy -> negative pressure -> NaN sound speed
+        s.set_state(U)
+        with pytest.raises(W.WenofdError) as e:
+            s.calculate_delta_t()
+        assert e.value.code == W.E_NAN_DT
+
+
+@pytest.mark.parametrize("n", [2048, 4096])
+@pytest.mark.parametrize("tc", [0, 2])
+def test_uniform_flow_stays_bitwise_uniform_full_size(n, tc):
+    """At the bandwidth-scale sizes: a uniform state is a steady solution and
+    every stencil sees identical data, so the state must not change by a bit."""
+    rho, u, v, p = 1.3, 0.2, -0.6, 0.8
+    E = p / 0.4 + 0.5 * rho * (u * u + v * v)
+    with solver(tc, 1, 2, 0.6, n, n, TEND=10) as s:
+        U0 = np.empty((4, n, n))
+        U0[0], U0[1], U0[2], U0[3] = rho, rho * u, rho * v, E
+        s.set_state(U0, W.LAYOUT_SOA)
+        for _ in range(2):
+            s.step()
+        U = s.get_state(W.LAYOUT_SOA)
+    assert np.array_equal(U, U0)
+
+
+def test_periodic_twin_nodes_and_conservation_2048():
+    """Periodic grids store the end point twice. With equal twins the two
+    nodes see identical stencils, so they stay bitwise equal, and the flux
+    differences telescope: mass, momentum and energy over the unique nodes
+    are conserved to rounding."""
+    n = 2048
+    with solver(4, 1, 2, 0.6, n, n, TEND=10) as s:
+        U = s.get_state()
+        U[-1, :] = U[0, :]
+        U[:, -1] = U[:, 0]
+        s.set_state(U)
+        m0 = U[:-1, :-1].sum(axis=(0, 1))
+        for _ in range(5):
+            s.step()
+        U = s.get_state()
+    assert np.array_equal(U[-1, :], U[0, :])
+    assert np.array_equal(U[:, -1], U[:, 0])
+    m1 = U[:-1, :-1].sum(axis=(0, 1))
+    assert np.all(np.abs(m1 - m0) <= 1e-12 * np.abs(U[:-1, :-1]).sum(axis=(0, 1)))
+
+
+def test_diagonal_symmetry_2048():
+    """The four-quadrant states are symmetric under x <-> y with u <-> v; the x
+    and y sweeps use mirrored formulas, so the solution stays symmetric to
+    rounding."""
+    n = 2048
+    with solver(5, 1, 2, 0.6, n, n) as s:
+        for _ in range(10):
+            s.step()
+        U = s.get_state()
+    T = np.transpose(U, (1, 0, 2))[..., [0, 2, 1, 3]]
+    assert relmax(U, T) < 1e-10
