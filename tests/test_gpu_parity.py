@@ -171,8 +171,10 @@ def test_smooth_convergence_table(golden, scheme):
     for a, b in zip(ns[:-1], ns[1:]):
         order = math.log(res[a][1] / res[b][1]) / math.log(b / a)
         assert 4.8 < order < 5.4, (a, b, order)
+    # the reference's own run() (makefile build, FMA-contracted) at N = 100: the
+    # norms are errors of rho ~ 1, so they agree to the FMA floor in absolute terms
     ref = golden("run_accuracy")[f"smooth_s{scheme}_n100_cfl0.05"]
-    assert np.allclose(res[100], ref[1:], rtol=1e-9, atol=0)
+    assert np.all(np.abs(res[100] - ref[1:]) <= 1e-12), (res[100], ref[1:])
 
 
 def test_seeded_quadrant_initial_state():
