@@ -44,16 +44,15 @@ def kernel_algorithmic(kind, counts, scheme, nx, ny, rk_first=False):
         faces, points = ny * (nx + 1), ny * (nx + 6)
         flops = faces * counts[f"face_x_{sname}"]["flops"] + points * counts["split_x"]["flops"]
         nbytes = points * 32 + faces * 32
-    elif kind == "flux_y":
-        faces, points = nx * (ny + 1), nx * (ny + 6)
-        flops = faces * counts[f"face_y_{sname}"]["flops"] + points * counts["split_y"]["flops"]
-        nbytes = points * 32 + faces * 32
-    elif kind == "rk_update":
-        flops = nx * ny * 4 * counts["cell_var_stage"]["flops"]
-        nbytes = nx * ny * (32 + 32 + 32 + 32 + 32)  # Fx, Fy, U, Un (or Un written), U written
-    elif kind == "delta_t":
-        flops = nx * ny * counts["cell_dt"]["flops"]
-        nbytes = nx * ny * 32
+    elif kind == "flux_y_rk":
+        faces, points, cells = nx * (ny + 1), nx * (ny + 6), nx * ny
+        flops = (faces * counts[f"face_y_{sname}"]["flops"] + points * counts["split_y"]["flops"]
+                 + cells * 4 * counts["cell_var_stage"]["flops"])
+        # U stencil points, x-face fluxes, Un, updated U
+        nbytes = points * 32 + ny * (nx + 1) * 32 + cells * 32 + cells * 32
+    elif kind == "delta_t":  # after the first step: folds the partial minima of the last stage
+        flops = 0
+        nbytes = 0
     else:  # ghost_fill
         flops = 0
         nbytes = 8 * (nx + ny) * 64

@@ -13,6 +13,7 @@
 #include <string>
 #include <sys/stat.h>
 #include <vector>
+#include <algorithm>
 
 #include "../../include/wenofd_amd.h"
 #include "wenofd_host.hpp"
@@ -52,14 +53,18 @@ struct wenofd_solver {
     Bc bc{};
     int device = 0;
     hipStream_t stream = nullptr;
-    double* U = nullptr;        // 4 state planes incl. ghosts
-    double* Un = nullptr;       // RK base state
+    // Three state buffers (4 planes incl. ghosts each): U is the current
+    // state; Ua and Ub hold intermediate RK stages. Stages never copy: RK3 runs
+    // U -> Ua -> Ub -> U with U doubling as the RK base state Un.
+    double* U = nullptr;
+    double* Ua = nullptr;
+    double* Ub = nullptr;
     double* Fx = nullptr;       // x-face fluxes
-    double* Fy = nullptr;       // y-face fluxes
     double* rhs = nullptr;      // rhs in the state layout (allocated on first get_flux)
     double* special = nullptr;  // SPECIAL boundary values [4][2 nx + 2 ny]
-    double* dt_part = nullptr;
+    double* dt_part = nullptr;  // partial CFL minima (standalone kernel or last RK stage)
     int* dt_part_nan = nullptr;
+    int dt_parts = 0;           // > 0: the partials describe the current state
     double* dt_dev = nullptr;   // {min, nan flag}
     double* dt_host = nullptr;  // pinned copy of dt_dev
     std::vector<double> xc, yc;
@@ -126,53 +131,70 @@ int set_dev(wenofd_solver* h) {
     return 0;
 }
 
-int fill_ghosts(wenofd_solver* h) {
-    return timed(h, WENOFD_K_GHOST, [&] { return launch_ghost_fill(h->U, h->special, h->g, h->bc, h->stream); });
-}
 
-// One RK stage on the current state: ghost fill, x and y face fluxes, fused
-// rhs + update.
-int stage(wenofd_solver* h, const RkCoef& k, double* rhs_out) {
+// One RK stage: ghost fill of `in`, x-face fluxes, then y faces fused with the
+// rhs and the update into `out` (which also reduces the next CFL step when
+// last_stage is set).
+int stage(wenofd_solver* h, double* in, const double* un, double* out, const RkCoef& k, double* rhs_out,
+          bool last_stage) {
     int rc;
-    if ((rc = fill_ghosts(h))) return rc;
-    if ((rc = timed(h, WENOFD_K_FLUX_X,
-                    [&] { return launch_flux_x(h->opt.scheme, h->U, h->Fx, h->g, h->stream); })))
+    if ((rc = timed(h, WENOFD_K_GHOST, [&] { return launch_ghost_fill(in, h->special, h->g, h->bc, h->stream); })))
         return rc;
-    if ((rc = timed(h, WENOFD_K_FLUX_Y,
-                    [&] { return launch_flux_y(h->opt.scheme, h->U, h->Fy, h->g, h->stream); })))
+    if ((rc = timed(h, WENOFD_K_FLUX_X, [&] { return launch_flux_x(h->opt.scheme, in, h->Fx, h->g, h->stream); })))
         return rc;
-    return timed(h, WENOFD_K_RK,
-                 [&] { return launch_rk_update(h->U, h->Un, h->Fx, h->Fy, rhs_out, h->g, k, h->stream); });
+    if ((rc = timed(h, WENOFD_K_FLUX_Y, [&] {
+             return launch_flux_y_rk(h->opt.scheme, in, h->Fx, un, out, rhs_out, h->g, k, h->opt.cfl,
+                                     last_stage ? h->dt_part : nullptr, last_stage ? h->dt_part_nan : nullptr,
+                                     h->stream);
+         })))
+        return rc;
+    if (last_stage) {
+        const dim3 gr = flux_y_rk_grid(h->g);
+        h->dt_parts = (int)(gr.x * gr.y);
+    }
+    return 0;
 }
 
 int run_rk(wenofd_solver* h, int rkmeth, double dt) {
     const int grav = h->opt.testcase == kRTI ? 1 : 0;
     int rc;
-    if (rkmeth == 0) {  // RunRK1, src/WENOFD.cpp:301-311
+    h->dt_parts = 0;
+    if (rkmeth == 0) {  // RunRK1, src/WENOFD.cpp:301-311: U += dt rhs
         RkCoef k{kRk1, grav, 0.0, 1.0, dt};
-        return stage(h, k, nullptr);
+        if ((rc = stage(h, h->U, nullptr, h->Ua, k, nullptr, true))) return rc;
+        std::swap(h->U, h->Ua);
+        return 0;
     }
-    if (rkmeth == 1) {  // RunRK2, :312-351
-        RkCoef k1{kRkFirst, grav, 1.0, 0.0, 1.0 * dt};
+    if (rkmeth == 1) {  // RunRK2, :312-351 (Un = U)
+        RkCoef k1{kRkStage, grav, 1.0, 0.0, 1.0 * dt};
         RkCoef k2{kRkStage, grav, 0.5, 0.5, 0.5 * dt};
-        if ((rc = stage(h, k1, nullptr))) return rc;
-        return stage(h, k2, nullptr);
+        if ((rc = stage(h, h->U, h->U, h->Ua, k1, nullptr, false))) return rc;
+        return stage(h, h->Ua, h->U, h->U, k2, nullptr, true);
     }
-    if (rkmeth == 2) {  // RunRK3, :352-396
-        RkCoef k1{kRkFirst, grav, 1.0, 0.0, 1.0 * dt};
+    if (rkmeth == 2) {  // RunRK3, :352-396 (Un = U)
+        RkCoef k1{kRkStage, grav, 1.0, 0.0, 1.0 * dt};
         RkCoef k2{kRkStage, grav, 3.0 / 4.0, 1.0 / 4.0, (1.0 / 4.0) * dt};
         RkCoef k3{kRkStage, grav, 1.0 / 3.0, 2.0 / 3.0, (2.0 / 3.0) * dt};
-        if ((rc = stage(h, k1, nullptr))) return rc;
-        if ((rc = stage(h, k2, nullptr))) return rc;
-        return stage(h, k3, nullptr);
+        if ((rc = stage(h, h->U, h->U, h->Ua, k1, nullptr, false))) return rc;
+        if ((rc = stage(h, h->Ua, h->U, h->Ub, k2, nullptr, false))) return rc;
+        return stage(h, h->Ub, h->U, h->U, k3, nullptr, true);
     }
     return fail(WENOFD_E_BAD_ENUM, "RKMETH must be 0, 1 or 2");
 }
 
+// calculateDeltaT: folds the partial minima the last RK stage left for the
+// current state, or computes them first (after creation or a state upload).
 int delta_t(wenofd_solver* h, double* dt) {
-    int rc = timed(h, WENOFD_K_DT, [&] {
-        return launch_dt(h->U, h->g, h->opt.cfl, h->dt_part, h->dt_part_nan, kDtBlocks, h->dt_dev, h->stream);
-    });
+    int rc;
+    if (h->dt_parts > 0) {
+        const int n = h->dt_parts;
+        rc = timed(h, WENOFD_K_DT,
+                   [&] { return launch_dt_final(h->dt_part, h->dt_part_nan, n, h->dt_dev, h->stream); });
+    } else {
+        rc = timed(h, WENOFD_K_DT, [&] {
+            return launch_dt(h->U, h->g, h->opt.cfl, h->dt_part, h->dt_part_nan, kDtBlocks, h->dt_dev, h->stream);
+        });
+    }
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(h->dt_host, h->dt_dev, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -210,9 +232,10 @@ int upload_initial(wenofd_solver* h) {
             h->cs.conserved0(h->xc[i], h->yc[j], Uc);
             for (int r = 0; r < 4; ++r) full[r * h->g.plane + (size_t)j * h->g.pitch + i] = Uc[r];
         }
-    HIP_TRY(hipMemcpyAsync(h->U, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->Un, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    for (double* b : {h->U, h->Ua, h->Ub})
+        HIP_TRY(hipMemcpyAsync(b, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    h->dt_parts = 0;
     h->now = 0;
     h->steps = 0;
     return 0;
@@ -265,13 +288,15 @@ int create_impl(const std::string& text, int device, wenofd_solver** out) {
     if ((rc = set_dev(h))) return cleanup_fail(rc);
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&h->U, h->state_doubles() * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&h->Un, h->state_doubles() * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&h->Ua, h->state_doubles() * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&h->Ub, h->state_doubles() * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&h->Fx, 4 * (size_t)g.fx_plane * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&h->Fy, 4 * (size_t)g.fy_plane * sizeof(double));
     const int ns = 2 * (g.nx + g.ny);
     if (e == hipSuccess) e = hipMalloc(&h->special, 4 * (size_t)ns * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&h->dt_part, kDtBlocks * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&h->dt_part_nan, kDtBlocks * sizeof(int));
+    const dim3 fyg = flux_y_rk_grid(g);
+    const size_t nparts = std::max<size_t>(kDtBlocks, (size_t)fyg.x * fyg.y);
+    if (e == hipSuccess) e = hipMalloc(&h->dt_part, nparts * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&h->dt_part_nan, nparts * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&h->dt_dev, 2 * sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc(&h->dt_host, 2 * sizeof(double), hipHostMallocDefault);
     if (e != hipSuccess) return cleanup_fail(fail(WENOFD_E_HIP, std::string("allocation: ") + hipGetErrorString(e)));
@@ -300,7 +325,6 @@ int create_impl(const std::string& text, int device, wenofd_solver** out) {
     }
     e = hipMemcpy(h->special, sp.data(), sp.size() * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemsetAsync(h->Fx, 0, 4 * (size_t)g.fx_plane * sizeof(double), h->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(h->Fy, 0, 4 * (size_t)g.fy_plane * sizeof(double), h->stream);
     if (e != hipSuccess) return cleanup_fail(fail(WENOFD_E_HIP, std::string("upload: ") + hipGetErrorString(e)));
     if ((rc = upload_initial(h))) return cleanup_fail(rc);
     *out = h;
@@ -417,9 +441,9 @@ int wenofd_destroy(wenofd_solver* h) {
     }
     for (auto e : h->event_pool) (void)hipEventDestroy(e);
     (void)hipFree(h->U);
-    (void)hipFree(h->Un);
+    (void)hipFree(h->Ua);
+    (void)hipFree(h->Ub);
     (void)hipFree(h->Fx);
-    (void)hipFree(h->Fy);
     (void)hipFree(h->rhs);
     (void)hipFree(h->special);
     (void)hipFree(h->dt_part);
@@ -452,7 +476,8 @@ int wenofd_run_rk(wenofd_solver* h, int rkmeth, double dt) {
 int wenofd_set_boundary(wenofd_solver* h) {
     if (!h) return fail(WENOFD_E_ARG, "null handle");
     int rc = set_dev(h);
-    return rc ? rc : fill_ghosts(h);
+    return rc ? rc
+              : timed(h, WENOFD_K_GHOST, [&] { return launch_ghost_fill(h->U, h->special, h->g, h->bc, h->stream); });
 }
 
 int wenofd_get_flux(wenofd_solver* h) {
@@ -464,7 +489,7 @@ int wenofd_get_flux(wenofd_solver* h) {
         HIP_TRY(hipMemsetAsync(h->rhs, 0, h->state_doubles() * sizeof(double), h->stream));
     }
     RkCoef k{kRkNone, h->opt.testcase == kRTI ? 1 : 0, 0.0, 0.0, 0.0};
-    return stage(h, k, h->rhs);
+    return stage(h, h->U, nullptr, nullptr, k, h->rhs, false);
 }
 
 static int copy_out(wenofd_solver* h, const double* dev, double* host, int layout) {
@@ -540,6 +565,7 @@ int wenofd_set_state(wenofd_solver* h, const double* host, int layout) {
     }
     HIP_TRY(hipMemcpyAsync(h->U, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    h->dt_parts = 0;
     return 0;
 }
 

@@ -7,7 +7,7 @@
 // Face fluxes: Fx[v][j-4][f-4] for faces f in [4, nx+4] of interior rows
 // (face f sits between points f-1 and f), Fy[v][f-4][i-4] likewise along y.
 //
-// One RK stage = ghost fill -> x faces -> y faces -> fused rhs + RK update.
+// One RK stage = ghost fill -> x faces -> y faces fused with rhs + RK update.
 // All arithmetic is FP64 in the reference's operation order (wenofd_math.hpp).
 
 #include <hip/hip_runtime.h>
@@ -134,27 +134,47 @@ __global__ __launch_bounds__(kXT) void k_flux_x(const double* __restrict__ U, do
 }
 
 // ---------------------------------------------------------------------------
-// y faces. One block = 64 interior columns x kYF faces along y, marched in
-// steps of 4 face rows (256 threads = 64 columns x 4 rows). The LLF split of
-// the 9 point rows one step reads lives in an LDS ring; each step adds 4 new
-// rows, so every point is split once per block (plus the 5-row lead-in).
+// y faces fused with the rhs and the RK stage update.
+//
+// One block = 64 interior columns x 63 interior cells along y (256 threads =
+// 64 columns x 4 rows). It marches 16 steps of 4 face rows: each step splits 4
+// new point rows into a 9-row LDS ring (so every point is split once per
+// block), builds 4 face rows into a 5-row LDS face buffer, and completes 4
+// cells (the lowest one with the face row carried from the previous step).
+// 16 steps x 4 faces = 64 faces = the 63 cells' faces, none computed twice.
+//
+// Per cell: rhs = x term (faces read from Fx) + y term [+ RTI source], then
+//   kRkNone:  rhs only (rhs_out)           kRk1:   out = in + dt rhs
+//   kRkStage: out = (a Un + b in) + (c dt) rhs
+// `out` is never `in` (neighbouring blocks still read `in`); it may be `Un`,
+// which is read and written only at the thread's own cell. With dt_part set,
+// the block also reduces the local CFL step of its updated cells (the next
+// calculateDeltaT) into dt_part[block] / dt_nan[block].
 // ---------------------------------------------------------------------------
-constexpr int kYC = 64;    // columns per block
-constexpr int kYR = 4;     // face rows per step
-constexpr int kYF = 64;    // faces per block along y
-constexpr int kRing = 9;   // ring rows: a step reads 9 consecutive point rows
+constexpr int kYC = 64;                   // columns per block
+constexpr int kYR = 4;                    // face rows per step
+constexpr int kYSteps = 16;               // steps per block
+constexpr int kYCells = kYR * kYSteps - 1;  // cells per block along y (63)
+constexpr int kRing = 9;                  // split ring rows: a step reads 9 point rows
+constexpr int kFaceRows = kYR + 1;        // face rows held: 4 new + 1 carried
 
 template <int S>
-__global__ __launch_bounds__(kYC* kYR) void k_flux_y(const double* __restrict__ U, double* __restrict__ Fy, Geom g) {
+__global__ __launch_bounds__(kYC* kYR) void k_flux_y_rk(const double* __restrict__ Uin, const double* __restrict__ Fx,
+                                                        const double* Un, double* Uout, double* __restrict__ rhs_out,
+                                                        Geom g, RkCoef k, double cfl, double* __restrict__ dt_part,
+                                                        int* __restrict__ dt_nan) {
     __shared__ double sP[4][kRing][kYC];
     __shared__ double sM[4][kRing][kYC];
+    __shared__ double sF[4][kFaceRows][kYC];
+    __shared__ double sMin[kYR];
+    __shared__ int sNan[kYR];
     const int tx = threadIdx.x % kYC;
     const int ty = threadIdx.x / kYC;
     const int i = 4 + blockIdx.x * kYC + tx;
     const bool col_ok = i < g.nx + 4;
-    const int F0 = 4 + blockIdx.y * kYF;                  // first face of the block
-    const int F1 = min(F0 + kYF, g.ny + 5);               // one past the last face
-    const int base = F0 - 3;                              // point row held in ring slot 0 at start
+    const int J0 = 4 + blockIdx.y * kYCells;  // first cell (and first face) of the block
+    const int base = J0 - 3;                  // point row of ring slot 0
+    const int Jend = min(J0 + kYCells, g.ny + 4);  // one past the block's last cell
 
     auto load_row = [&](int prow) {
         const int slot = (prow - base) % kRing;
@@ -162,7 +182,7 @@ __global__ __launch_bounds__(kYC* kYR) void k_flux_y(const double* __restrict__ 
             double u[4], fm[4], fp[4];
             const size_t o = (size_t)prow * g.pitch + i;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) u[r] = U[r * g.plane + o];
+            for (int r = 0; r < 4; ++r) u[r] = Uin[r * g.plane + o];
             flux_split<1>(u, g.gamma, fm, fp);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -171,81 +191,90 @@ __global__ __launch_bounds__(kYC* kYR) void k_flux_y(const double* __restrict__ 
             }
         }
     };
-    // lead-in: point rows F0-3 .. F0+1
-    for (int k = ty; k < 5; k += kYR) load_row(base + k);
-    for (int f = F0; f < F1; f += kYR) {
-        load_row(f + 2 + ty);  // rows f+2 .. f+5 complete the window [f-3, f+6)
+
+    double my_min = 1.0;
+    int my_nan = 0;
+    for (int k0 = ty; k0 < 5; k0 += kYR) load_row(base + k0);  // point rows J0-3 .. J0+1
+    for (int step = 0; step < kYSteps; ++step) {
+        const int f = J0 + kYR * step;  // first face row of this step
+        if (f > Jend) break;            // uniform across the block
+        load_row(f + 2 + ty);           // completes the window [f-3, f+6)
         __syncthreads();
         const int fr = f + ty;
-        if (col_ok && fr < F1) {
+        if (col_ok && fr <= Jend) {
             double Ul[4], Ur[4], Fh[4];
             const size_t ol = (size_t)(fr - 1) * g.pitch + i;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                Ul[r] = U[r * g.plane + ol];
-                Ur[r] = U[r * g.plane + ol + g.pitch];
+                Ul[r] = Uin[r * g.plane + ol];
+                Ur[r] = Uin[r * g.plane + ol + g.pitch];
             }
-            const int s0 = fr - 3 - base;  // ring position of point row fr-3
+            const int s0 = fr - 3 - base;
             face_flux<1, S>(
                 Ul, Ur, [&](int s, int r) { return sP[r][(s0 + s) % kRing][tx]; },
                 [&](int s, int r) { return sM[r][(s0 + s) % kRing][tx]; }, g.gamma, g.zp_lambda, Fh);
-            const size_t o = (size_t)(fr - 4) * g.fy_pitch + (i - 4);
+            const int fs = (fr - J0 + 1) % kFaceRows;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) Fy[r * g.fy_plane + o] = Fh[r];
+            for (int r = 0; r < 4; ++r) sF[r][fs][tx] = Fh[r];
         }
         __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------
-// rhs + RK stage, src/WENOFD.cpp:454-465 (rhs = x term + y term [+ source]),
-// :800-821 (RTI gravity), :301-396 (stage update):
-//   mode kRk1:     U = U + dt rhs
-//   mode kRkStage: U = (a Un + b U) + (c dt) rhs
-//   mode kRkFirst: Un = U, then as kRkStage
-// rhs_out (optional) receives rhs in the state layout (interior only).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_rk_update(double* __restrict__ U, double* __restrict__ Un,
-                                                   const double* __restrict__ Fx, const double* __restrict__ Fy,
-                                                   double* __restrict__ rhs_out, Geom g, RkCoef k) {
-    const int il = blockIdx.x * blockDim.x + threadIdx.x;
-    const int jl = blockIdx.y;
-    if (il >= g.nx) return;
-    const size_t o = (size_t)(jl + 4) * g.pitch + (il + 4);
-    const size_t ox = (size_t)jl * g.fx_pitch + il;
-    const size_t oy = (size_t)jl * g.fy_pitch + il;
-    double rhs[4], u[4];
+        const int j = f - 1 + ty;  // cell completed by this thread in this step
+        if (col_ok && j >= J0 && j < Jend) {
+            const int fb = (j - J0 + 1) % kFaceRows, ft = (j - J0 + 2) % kFaceRows;
+            const size_t o = (size_t)j * g.pitch + i;
+            const size_t ox = (size_t)(j - 4) * g.fx_pitch + (i - 4);
+            double rhs[4], u[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        rhs[r] = cell_rhs(Fx[r * g.fx_plane + ox], Fx[r * g.fx_plane + ox + 1], Fy[r * g.fy_plane + oy],
-                          Fy[r * g.fy_plane + oy + g.fy_pitch], g.dx, g.dy);
-        u[r] = U[r * g.plane + o];
-    }
-    if (k.gravity) {
-        rhs[3] = rhs[3] + u[2];
-        rhs[2] = rhs[2] + u[0];
-    }
-    if (rhs_out) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) rhs_out[r * g.plane + o] = rhs[r];
-    }
-    if (k.mode == kRkNone) return;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        double nu;
-        if (k.mode == kRk1) {
-            nu = u[r] + k.cdt * rhs[r];
-        } else {
-            double un;
-            if (k.mode == kRkFirst) {
-                un = u[r];
-                Un[r * g.plane + o] = un;
-            } else {
-                un = Un[r * g.plane + o];
+            for (int r = 0; r < 4; ++r) {
+                rhs[r] = cell_rhs(Fx[r * g.fx_plane + ox], Fx[r * g.fx_plane + ox + 1], sF[r][fb][tx], sF[r][ft][tx],
+                                  g.dx, g.dy);
+                u[r] = Uin[r * g.plane + o];
             }
-            nu = rk_combine(k.a, un, k.b, u[r], k.cdt, rhs[r]);
+            if (k.gravity) {  // src/WENOFD.cpp:812-813
+                rhs[3] = rhs[3] + u[2];
+                rhs[2] = rhs[2] + u[0];
+            }
+            if (rhs_out) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) rhs_out[r * g.plane + o] = rhs[r];
+            }
+            if (k.mode != kRkNone) {
+                double nu[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    nu[r] = (k.mode == kRk1) ? u[r] + k.cdt * rhs[r]
+                                             : rk_combine(k.a, Un[r * g.plane + o], k.b, u[r], k.cdt, rhs[r]);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Uout[r * g.plane + o] = nu[r];
+                if (dt_part) {
+                    const double t = local_dt(nu, g.gamma, cfl, g.dx, g.dy);
+                    if (t != t) my_nan = 1;
+                    my_min = (t < my_min) ? t : my_min;
+                }
+            }
         }
-        U[r * g.plane + o] = nu;
+    }
+    if (dt_part) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(my_min, off, 64);
+            my_min = (o < my_min) ? o : my_min;
+            my_nan |= __shfl_xor(my_nan, off, 64);
+        }
+        if (tx == 0) {
+            sMin[ty] = my_min;
+            sNan[ty] = my_nan;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int q = 1; q < kYR; ++q) {
+                my_min = (sMin[q] < my_min) ? sMin[q] : my_min;
+                my_nan |= sNan[q];
+            }
+            const int b = blockIdx.y * gridDim.x + blockIdx.x;
+            dt_part[b] = my_min;
+            dt_nan[b] = my_nan;
+        }
     }
 }
 
@@ -366,9 +395,10 @@ static void launch_fx_s(const double* U, double* Fx, const Geom& g, hipStream_t 
 }
 
 template <int S>
-static void launch_fy_s(const double* U, double* Fy, const Geom& g, hipStream_t s) {
-    dim3 grid((g.nx + kYC - 1) / kYC, (g.ny + 1 + kYF - 1) / kYF);
-    hipLaunchKernelGGL(k_flux_y<S>, grid, dim3(kYC * kYR), 0, s, U, Fy, g);
+static void launch_fyrk_s(const double* Uin, const double* Fx, const double* Un, double* Uout, double* rhs_out,
+                          const Geom& g, const RkCoef& k, double cfl, double* dt_part, int* dt_nan, hipStream_t s) {
+    hipLaunchKernelGGL(k_flux_y_rk<S>, flux_y_rk_grid(g), dim3(kYC * kYR), 0, s, Uin, Fx, Un, Uout, rhs_out, g, k, cfl,
+                       dt_part, dt_nan);
 }
 
 hipError_t launch_flux_x(int scheme, const double* U, double* Fx, const Geom& g, hipStream_t s) {
@@ -381,20 +411,22 @@ hipError_t launch_flux_x(int scheme, const double* U, double* Fx, const Geom& g,
     return hipGetLastError();
 }
 
-hipError_t launch_flux_y(int scheme, const double* U, double* Fy, const Geom& g, hipStream_t s) {
+dim3 flux_y_rk_grid(const Geom& g) { return dim3((g.nx + kYC - 1) / kYC, (g.ny + kYCells - 1) / kYCells); }
+
+hipError_t launch_flux_y_rk(int scheme, const double* Uin, const double* Fx, const double* Un, double* Uout,
+                            double* rhs_out, const Geom& g, const RkCoef& k, double cfl, double* dt_part, int* dt_nan,
+                            hipStream_t s) {
     switch (scheme) {
-        case kJS: launch_fy_s<kJS>(U, Fy, g, s); break;
-        case kZ: launch_fy_s<kZ>(U, Fy, g, s); break;
-        case kZP: launch_fy_s<kZP>(U, Fy, g, s); break;
-        default: launch_fy_s<kZPI>(U, Fy, g, s); break;
+        case kJS: launch_fyrk_s<kJS>(Uin, Fx, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
+        case kZ: launch_fyrk_s<kZ>(Uin, Fx, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
+        case kZP: launch_fyrk_s<kZP>(Uin, Fx, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
+        default: launch_fyrk_s<kZPI>(Uin, Fx, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_rk_update(double* U, double* Un, const double* Fx, const double* Fy, double* rhs_out,
-                            const Geom& g, const RkCoef& k, hipStream_t s) {
-    dim3 grid((g.nx + 255) / 256, g.ny);
-    hipLaunchKernelGGL(k_rk_update, grid, dim3(256), 0, s, U, Un, Fx, Fy, rhs_out, g, k);
+hipError_t launch_dt_final(const double* part, const int* part_nan, int nparts, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, part, part_nan, nparts, out);
     return hipGetLastError();
 }
 

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libwenofd_amd.so")
 OK, E_CONFIG, E_BAD_ENUM, E_NAN_DT, E_HIP, E_ARG, E_IO = range(7)
 LAYOUT_REF, LAYOUT_SOA, LAYOUT_REF_GHOSTED = 0, 1, 2
 K_GHOST, K_FLUX_X, K_FLUX_Y, K_RK, K_DT, K_COUNT = range(6)
-KERNEL_NAMES = ("ghost_fill", "flux_x", "flux_y", "rk_update", "delta_t")
+KERNEL_NAMES = ("ghost_fill", "flux_x", "flux_y_rk", "unused", "delta_t")
 
 # Exported symbols declared in include/wenofd_amd.h.
 EXPORTS = (
