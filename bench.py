@@ -40,16 +40,16 @@ def flop_counts():
 def kernel_algorithmic(kind, counts, scheme, nx, ny, rk_first=False):
     """Algorithmic FLOPs and HBM bytes of one launch (per-unit figures x units)."""
     sname = {0: "js", 1: "z", 2: "zp", 3: "zpi"}[scheme]
-    if kind == "flux_x":
-        faces, points = ny * (nx + 1), ny * (nx + 6)
-        flops = faces * counts[f"face_x_{sname}"]["flops"] + points * counts["split_x"]["flops"]
-        nbytes = points * 32 + faces * 32
-    elif kind == "flux_y_rk":
-        faces, points, cells = nx * (ny + 1), nx * (ny + 6), nx * ny
-        flops = (faces * counts[f"face_y_{sname}"]["flops"] + points * counts["split_y"]["flops"]
+    if kind == "flux_x_rk":
+        faces, points, cells = ny * (nx + 1), ny * (nx + 6), nx * ny
+        flops = (faces * counts[f"face_x_{sname}"]["flops"] + points * counts["split_x"]["flops"]
                  + cells * 4 * counts["cell_var_stage"]["flops"])
-        # U stencil points, x-face fluxes, Un, updated U
-        nbytes = points * 32 + ny * (nx + 1) * 32 + cells * 32 + cells * 32
+        # U stencil points, y-face fluxes, Un, updated U
+        nbytes = points * 32 + nx * (ny + 1) * 32 + cells * 32 + cells * 32
+    elif kind == "flux_y":
+        faces, points = nx * (ny + 1), nx * (ny + 6)
+        flops = faces * counts[f"face_y_{sname}"]["flops"] + points * counts["split_y"]["flops"]
+        nbytes = points * 32 + faces * 32
     elif kind == "delta_t":  # after the first step: folds the partial minima of the last stage
         flops = 0
         nbytes = 0

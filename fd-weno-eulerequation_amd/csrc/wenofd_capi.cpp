@@ -15,7 +15,7 @@
 #include <vector>
 #include <algorithm>
 
-#include "../../include/wenofd_amd.h"
+#include "wenofd_amd.h"
 #include "wenofd_host.hpp"
 #include "wenofd_kernels.hpp"
 
@@ -59,7 +59,7 @@ struct wenofd_solver {
     double* U = nullptr;
     double* Ua = nullptr;
     double* Ub = nullptr;
-    double* Fx = nullptr;       // x-face fluxes
+    double* Fy = nullptr;       // y-face fluxes
     double* rhs = nullptr;      // rhs in the state layout (allocated on first get_flux)
     double* special = nullptr;  // SPECIAL boundary values [4][2 nx + 2 ny]
     double* dt_part = nullptr;  // partial CFL minima (standalone kernel or last RK stage)
@@ -132,7 +132,7 @@ int set_dev(wenofd_solver* h) {
 }
 
 
-// One RK stage: ghost fill of `in`, x-face fluxes, then y faces fused with the
+// One RK stage: ghost fill of `in`, y-face fluxes, then x faces fused with the
 // rhs and the update into `out` (which also reduces the next CFL step when
 // last_stage is set).
 int stage(wenofd_solver* h, double* in, const double* un, double* out, const RkCoef& k, double* rhs_out,
@@ -140,16 +140,16 @@ int stage(wenofd_solver* h, double* in, const double* un, double* out, const RkC
     int rc;
     if ((rc = timed(h, WENOFD_K_GHOST, [&] { return launch_ghost_fill(in, h->special, h->g, h->bc, h->stream); })))
         return rc;
-    if ((rc = timed(h, WENOFD_K_FLUX_X, [&] { return launch_flux_x(h->opt.scheme, in, h->Fx, h->g, h->stream); })))
+    if ((rc = timed(h, WENOFD_K_FLUX_Y, [&] { return launch_flux_y(h->opt.scheme, in, h->Fy, h->g, h->stream); })))
         return rc;
-    if ((rc = timed(h, WENOFD_K_FLUX_Y, [&] {
-             return launch_flux_y_rk(h->opt.scheme, in, h->Fx, un, out, rhs_out, h->g, k, h->opt.cfl,
+    if ((rc = timed(h, WENOFD_K_FLUX_X, [&] {
+             return launch_flux_x_rk(h->opt.scheme, in, h->Fy, un, out, rhs_out, h->g, k, h->opt.cfl,
                                      last_stage ? h->dt_part : nullptr, last_stage ? h->dt_part_nan : nullptr,
                                      h->stream);
          })))
         return rc;
     if (last_stage) {
-        const dim3 gr = flux_y_rk_grid(h->g);
+        const dim3 gr = flux_x_rk_grid(h->g);
         h->dt_parts = (int)(gr.x * gr.y);
     }
     return 0;
@@ -290,11 +290,11 @@ int create_impl(const std::string& text, int device, wenofd_solver** out) {
     if (e == hipSuccess) e = hipMalloc(&h->U, h->state_doubles() * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&h->Ua, h->state_doubles() * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&h->Ub, h->state_doubles() * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&h->Fx, 4 * (size_t)g.fx_plane * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&h->Fy, 4 * (size_t)g.fy_plane * sizeof(double));
     const int ns = 2 * (g.nx + g.ny);
     if (e == hipSuccess) e = hipMalloc(&h->special, 4 * (size_t)ns * sizeof(double));
-    const dim3 fyg = flux_y_rk_grid(g);
-    const size_t nparts = std::max<size_t>(kDtBlocks, (size_t)fyg.x * fyg.y);
+    const dim3 fxg = flux_x_rk_grid(g);
+    const size_t nparts = std::max<size_t>(kDtBlocks, (size_t)fxg.x * fxg.y);
     if (e == hipSuccess) e = hipMalloc(&h->dt_part, nparts * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&h->dt_part_nan, nparts * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&h->dt_dev, 2 * sizeof(double));
@@ -324,7 +324,7 @@ int create_impl(const std::string& text, int device, wenofd_solver** out) {
         }
     }
     e = hipMemcpy(h->special, sp.data(), sp.size() * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemsetAsync(h->Fx, 0, 4 * (size_t)g.fx_plane * sizeof(double), h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->Fy, 0, 4 * (size_t)g.fy_plane * sizeof(double), h->stream);
     if (e != hipSuccess) return cleanup_fail(fail(WENOFD_E_HIP, std::string("upload: ") + hipGetErrorString(e)));
     if ((rc = upload_initial(h))) return cleanup_fail(rc);
     *out = h;
@@ -443,7 +443,7 @@ int wenofd_destroy(wenofd_solver* h) {
     (void)hipFree(h->U);
     (void)hipFree(h->Ua);
     (void)hipFree(h->Ub);
-    (void)hipFree(h->Fx);
+    (void)hipFree(h->Fy);
     (void)hipFree(h->rhs);
     (void)hipFree(h->special);
     (void)hipFree(h->dt_part);

@@ -4,7 +4,7 @@
 #include <cstring>
 #include <sstream>
 
-#include "../../include/wenofd_amd.h"
+#include "wenofd_amd.h"
 #include "wenofd_host.hpp"
 #include "wenofd_kernels.hpp"
 

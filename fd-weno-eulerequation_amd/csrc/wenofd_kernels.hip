@@ -4,10 +4,10 @@
 //     U[v * plane + j * pitch + i],  v in 0..3 (rho, rho u, rho v, E),
 //     i in [0, nx+8) along x, j in [0, ny+8) along y, 4 ghost points per side,
 // interior i, j in [4, n+4) (the reference's node numbering, src/WENOFD.cpp:36-70).
-// Face fluxes: Fx[v][j-4][f-4] for faces f in [4, nx+4] of interior rows
-// (face f sits between points f-1 and f), Fy[v][f-4][i-4] likewise along y.
+// y-face fluxes: Fy[v][f-4][i-4] for faces f in [4, ny+4] of interior columns
+// (face f sits between points f-1 and f); x-face fluxes live only in LDS.
 //
-// One RK stage = ghost fill -> x faces -> y faces fused with rhs + RK update.
+// One RK stage = ghost fill -> y faces -> x faces fused with rhs + RK update.
 // All arithmetic is FP64 in the reference's operation order (wenofd_math.hpp).
 
 #include <hip/hip_runtime.h>
@@ -83,98 +83,28 @@ __global__ __launch_bounds__(256) void k_ghost_fill(double* __restrict__ U, cons
 }
 
 // ---------------------------------------------------------------------------
-// x faces. One block = 256 consecutive faces of one interior row; the LLF split
-// of the 261 points they read is computed once into LDS (component-major, so a
-// wave's 64 consecutive slots hit 64 distinct banks), then each thread builds
-// one face flux. Loads of U are coalesced along x.
+// y faces. One block = 64 interior columns x 64 faces along y, marched in
+// steps of 4 face rows (256 threads = 64 columns x 4 rows). The LLF split of
+// the 9 point rows one step reads lives in an LDS ring; each step adds 4 new
+// rows, so every point is split once per block (plus the 5-row lead-in).
+// Loads of U are coalesced along x.
 // ---------------------------------------------------------------------------
-constexpr int kXT = 256;           // faces per block
-constexpr int kXP = kXT + 5;       // points per block
-constexpr int kXS = kXT + 8;       // LDS row stride (slots)
+constexpr int kYC = 64;    // columns per block
+constexpr int kYR = 4;     // face rows per step
+constexpr int kYF = 64;    // faces per block along y
+constexpr int kRing = 9;   // ring rows: a step reads 9 consecutive point rows
 
 template <int S>
-__global__ __launch_bounds__(kXT) void k_flux_x(const double* __restrict__ U, double* __restrict__ Fx, Geom g) {
-    __shared__ double sP[4][kXS];
-    __shared__ double sM[4][kXS];
-    __shared__ double sU[4][kXS];
-    const int t = threadIdx.x;
-    const int j = blockIdx.y + 4;
-    const int f0 = 4 + blockIdx.x * kXT;
-    const double* row = U + (size_t)j * g.pitch;
-    for (int k = t; k < kXP; k += kXT) {
-        const int p = f0 - 3 + k;
-        if (p < g.nx + 8) {
-            double u[4], fm[4], fp[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) u[r] = row[r * g.plane + p];
-            flux_split<0>(u, g.gamma, fm, fp);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                sP[r][k] = fp[r];
-                sM[r][k] = fm[r];
-                sU[r][k] = u[r];
-            }
-        }
-    }
-    __syncthreads();
-    const int f = f0 + t;
-    if (f > g.nx + 4) return;
-    double Ul[4], Ur[4], Fh[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        Ul[r] = sU[r][t + 2];
-        Ur[r] = sU[r][t + 3];
-    }
-    face_flux<0, S>(
-        Ul, Ur, [&](int s, int r) { return sP[r][t + s]; }, [&](int s, int r) { return sM[r][t + s]; }, g.gamma,
-        g.zp_lambda, Fh);
-    const size_t o = (size_t)(j - 4) * g.fx_pitch + (f - 4);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) Fx[r * g.fx_plane + o] = Fh[r];
-}
-
-// ---------------------------------------------------------------------------
-// y faces fused with the rhs and the RK stage update.
-//
-// One block = 64 interior columns x 63 interior cells along y (256 threads =
-// 64 columns x 4 rows). It marches 16 steps of 4 face rows: each step splits 4
-// new point rows into a 9-row LDS ring (so every point is split once per
-// block), builds 4 face rows into a 5-row LDS face buffer, and completes 4
-// cells (the lowest one with the face row carried from the previous step).
-// 16 steps x 4 faces = 64 faces = the 63 cells' faces, none computed twice.
-//
-// Per cell: rhs = x term (faces read from Fx) + y term [+ RTI source], then
-//   kRkNone:  rhs only (rhs_out)           kRk1:   out = in + dt rhs
-//   kRkStage: out = (a Un + b in) + (c dt) rhs
-// `out` is never `in` (neighbouring blocks still read `in`); it may be `Un`,
-// which is read and written only at the thread's own cell. With dt_part set,
-// the block also reduces the local CFL step of its updated cells (the next
-// calculateDeltaT) into dt_part[block] / dt_nan[block].
-// ---------------------------------------------------------------------------
-constexpr int kYC = 64;                   // columns per block
-constexpr int kYR = 4;                    // face rows per step
-constexpr int kYSteps = 16;               // steps per block
-constexpr int kYCells = kYR * kYSteps - 1;  // cells per block along y (63)
-constexpr int kRing = 9;                  // split ring rows: a step reads 9 point rows
-constexpr int kFaceRows = kYR + 1;        // face rows held: 4 new + 1 carried
-
-template <int S>
-__global__ __launch_bounds__(kYC* kYR) void k_flux_y_rk(const double* __restrict__ Uin, const double* __restrict__ Fx,
-                                                        const double* Un, double* Uout, double* __restrict__ rhs_out,
-                                                        Geom g, RkCoef k, double cfl, double* __restrict__ dt_part,
-                                                        int* __restrict__ dt_nan) {
+__global__ __launch_bounds__(kYC* kYR) void k_flux_y(const double* __restrict__ U, double* __restrict__ Fy, Geom g) {
     __shared__ double sP[4][kRing][kYC];
     __shared__ double sM[4][kRing][kYC];
-    __shared__ double sF[4][kFaceRows][kYC];
-    __shared__ double sMin[kYR];
-    __shared__ int sNan[kYR];
     const int tx = threadIdx.x % kYC;
     const int ty = threadIdx.x / kYC;
     const int i = 4 + blockIdx.x * kYC + tx;
     const bool col_ok = i < g.nx + 4;
-    const int J0 = 4 + blockIdx.y * kYCells;  // first cell (and first face) of the block
-    const int base = J0 - 3;                  // point row of ring slot 0
-    const int Jend = min(J0 + kYCells, g.ny + 4);  // one past the block's last cell
+    const int F0 = 4 + blockIdx.y * kYF;     // first face of the block
+    const int F1 = min(F0 + kYF, g.ny + 5);  // one past the last face
+    const int base = F0 - 3;                 // point row held in ring slot 0
 
     auto load_row = [&](int prow) {
         const int slot = (prow - base) % kRing;
@@ -182,7 +112,7 @@ __global__ __launch_bounds__(kYC* kYR) void k_flux_y_rk(const double* __restrict
             double u[4], fm[4], fp[4];
             const size_t o = (size_t)prow * g.pitch + i;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) u[r] = Uin[r * g.plane + o];
+            for (int r = 0; r < 4; ++r) u[r] = U[r * g.plane + o];
             flux_split<1>(u, g.gamma, fm, fp);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -191,83 +121,156 @@ __global__ __launch_bounds__(kYC* kYR) void k_flux_y_rk(const double* __restrict
             }
         }
     };
-
-    double my_min = 1.0;
-    int my_nan = 0;
-    for (int k0 = ty; k0 < 5; k0 += kYR) load_row(base + k0);  // point rows J0-3 .. J0+1
-    for (int step = 0; step < kYSteps; ++step) {
-        const int f = J0 + kYR * step;  // first face row of this step
-        if (f > Jend) break;            // uniform across the block
-        load_row(f + 2 + ty);           // completes the window [f-3, f+6)
+    for (int k = ty; k < 5; k += kYR) load_row(base + k);  // lead-in: point rows F0-3 .. F0+1
+    for (int f = F0; f < F1; f += kYR) {
+        load_row(f + 2 + ty);  // rows f+2 .. f+5 complete the window [f-3, f+6)
         __syncthreads();
         const int fr = f + ty;
-        if (col_ok && fr <= Jend) {
+        if (col_ok && fr < F1) {
             double Ul[4], Ur[4], Fh[4];
             const size_t ol = (size_t)(fr - 1) * g.pitch + i;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                Ul[r] = Uin[r * g.plane + ol];
-                Ur[r] = Uin[r * g.plane + ol + g.pitch];
+                Ul[r] = U[r * g.plane + ol];
+                Ur[r] = U[r * g.plane + ol + g.pitch];
             }
-            const int s0 = fr - 3 - base;
-            face_flux<1, S>(
+            const int s0 = fr - 3 - base;  // ring position of point row fr-3
+            face_flux_dev<1, S>(
                 Ul, Ur, [&](int s, int r) { return sP[r][(s0 + s) % kRing][tx]; },
                 [&](int s, int r) { return sM[r][(s0 + s) % kRing][tx]; }, g.gamma, g.zp_lambda, Fh);
-            const int fs = (fr - J0 + 1) % kFaceRows;
+            const size_t o = (size_t)(fr - 4) * g.fy_pitch + (i - 4);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) sF[r][fs][tx] = Fh[r];
+            for (int r = 0; r < 4; ++r) Fy[r * g.fy_plane + o] = Fh[r];
         }
         __syncthreads();
-        const int j = f - 1 + ty;  // cell completed by this thread in this step
-        if (col_ok && j >= J0 && j < Jend) {
-            const int fb = (j - J0 + 1) % kFaceRows, ft = (j - J0 + 2) % kFaceRows;
-            const size_t o = (size_t)j * g.pitch + i;
-            const size_t ox = (size_t)(j - 4) * g.fx_pitch + (i - 4);
-            double rhs[4], u[4];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x faces fused with the rhs and the RK stage update.
+//
+// One block = 256 consecutive faces of one interior row = the 255 cells
+// between them (consecutive blocks overlap by one face). The LLF split of the
+// 261 points the faces read is computed once into LDS (component-major: a
+// wave's 64 consecutive slots hit 64 distinct banks), then each thread builds
+// one face flux into LDS and completes one cell:
+//   rhs = x term + y term (faces read from Fy) [+ RTI source], then
+//   kRkNone:  rhs only (rhs_out)           kRk1:   out = in + dt rhs
+//   kRkStage: out = (a Un + b in) + (c dt) rhs
+// `out` is never `in` (neighbouring blocks still read `in`); it may be `Un`,
+// which is read and written only at the thread's own cell. With dt_part set,
+// the block also reduces the local CFL step of its updated cells (the next
+// calculateDeltaT) into dt_part[block] / dt_nan[block].
+// ---------------------------------------------------------------------------
+constexpr int kXT = 256;      // faces per block
+constexpr int kXC = kXT - 1;  // cells per block
+constexpr int kXP = kXT + 5;  // points per block
+constexpr int kXS = kXT + 8;  // LDS row stride (slots)
+
+template <int S>
+__global__ __launch_bounds__(kXT) void k_flux_x_rk(const double* __restrict__ Uin, const double* __restrict__ Fy,
+                                                   const double* Un, double* Uout, double* __restrict__ rhs_out,
+                                                   Geom g, RkCoef k, double cfl, double* __restrict__ dt_part,
+                                                   int* __restrict__ dt_nan) {
+    __shared__ double sP[4][kXS];
+    __shared__ double sM[4][kXS];
+    __shared__ double sU[4][kXS];
+    __shared__ double sF[4][kXT];
+    __shared__ double sMin[kXT / 64];
+    __shared__ int sNan[kXT / 64];
+    const int t = threadIdx.x;
+    const int j = blockIdx.y + 4;
+    const int f0 = 4 + blockIdx.x * kXC;  // first face = first cell of the block
+    const double* row = Uin + (size_t)j * g.pitch;
+    for (int q = t; q < kXP; q += kXT) {
+        const int p = f0 - 3 + q;  // point p sits in slot p - f0 + 3
+        if (p < g.nx + 8) {
+            double u[4], fm[4], fp[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) u[r] = row[r * g.plane + p];
+            flux_split<0>(u, g.gamma, fm, fp);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                rhs[r] = cell_rhs(Fx[r * g.fx_plane + ox], Fx[r * g.fx_plane + ox + 1], sF[r][fb][tx], sF[r][ft][tx],
-                                  g.dx, g.dy);
-                u[r] = Uin[r * g.plane + o];
+                sP[r][q] = fp[r];
+                sM[r][q] = fm[r];
+                sU[r][q] = u[r];
             }
-            if (k.gravity) {  // src/WENOFD.cpp:812-813
-                rhs[3] = rhs[3] + u[2];
-                rhs[2] = rhs[2] + u[0];
-            }
-            if (rhs_out) {
+        }
+    }
+    __syncthreads();
+    const int f = f0 + t;
+    if (f <= g.nx + 4) {
+        double Ul[4], Ur[4], Fh[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) rhs_out[r * g.plane + o] = rhs[r];
-            }
-            if (k.mode != kRkNone) {
-                double nu[4];
+        for (int r = 0; r < 4; ++r) {
+            Ul[r] = sU[r][t + 2];
+            Ur[r] = sU[r][t + 3];
+        }
+        face_flux_dev<0, S>(
+            Ul, Ur, [&](int s, int r) { return sP[r][t + s]; }, [&](int s, int r) { return sM[r][t + s]; },
+            g.gamma, g.zp_lambda, Fh);
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    nu[r] = (k.mode == kRk1) ? u[r] + k.cdt * rhs[r]
-                                             : rk_combine(k.a, Un[r * g.plane + o], k.b, u[r], k.cdt, rhs[r]);
+        for (int r = 0; r < 4; ++r) sF[r][t] = Fh[r];
+    }
+    const int i = f0 + t;  // this thread's cell
+    const bool cell_ok = t < kXC && i < g.nx + 4;
+    const size_t o = (size_t)j * g.pitch + i;
+    const size_t oy = (size_t)(j - 4) * g.fy_pitch + (i - 4);
+    double fyb[4], fyt[4], unb[4];
+    if (cell_ok) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) Uout[r * g.plane + o] = nu[r];
-                if (dt_part) {
-                    const double t = local_dt(nu, g.gamma, cfl, g.dx, g.dy);
-                    if (t != t) my_nan = 1;
-                    my_min = (t < my_min) ? t : my_min;
-                }
+        for (int r = 0; r < 4; ++r) {
+            fyb[r] = Fy[r * g.fy_plane + oy];
+            fyt[r] = Fy[r * g.fy_plane + oy + g.fy_pitch];
+            unb[r] = (k.mode == kRkStage) ? Un[r * g.plane + o] : 0.0;
+        }
+    }
+    __syncthreads();
+    double my_min = 1.0;
+    int my_nan = 0;
+    if (cell_ok) {
+        double rhs[4], u[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            rhs[r] = cell_rhs(sF[r][t], sF[r][t + 1], fyb[r], fyt[r], g.dx, g.dy);
+            u[r] = sU[r][t + 3];
+        }
+        if (k.gravity) {  // src/WENOFD.cpp:812-813
+            rhs[3] = rhs[3] + u[2];
+            rhs[2] = rhs[2] + u[0];
+        }
+        if (rhs_out) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) rhs_out[r * g.plane + o] = rhs[r];
+        }
+        if (k.mode != kRkNone) {
+            double nu[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                nu[r] = (k.mode == kRk1) ? u[r] + k.cdt * rhs[r] : rk_combine(k.a, unb[r], k.b, u[r], k.cdt, rhs[r]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Uout[r * g.plane + o] = nu[r];
+            if (dt_part) {
+                const double tl = local_dt(nu, g.gamma, cfl, g.dx, g.dy);
+                if (tl != tl) my_nan = 1;
+                my_min = (tl < my_min) ? tl : my_min;
             }
         }
     }
     if (dt_part) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
-            const double o = __shfl_xor(my_min, off, 64);
-            my_min = (o < my_min) ? o : my_min;
+            const double ov = __shfl_xor(my_min, off, 64);
+            my_min = (ov < my_min) ? ov : my_min;
             my_nan |= __shfl_xor(my_nan, off, 64);
         }
-        if (tx == 0) {
-            sMin[ty] = my_min;
-            sNan[ty] = my_nan;
+        if (t % 64 == 0) {
+            sMin[t / 64] = my_min;
+            sNan[t / 64] = my_nan;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int q = 1; q < kYR; ++q) {
+        if (t == 0) {
+            for (int q = 1; q < kXT / 64; ++q) {
                 my_min = (sMin[q] < my_min) ? sMin[q] : my_min;
                 my_nan |= sNan[q];
             }
@@ -329,15 +332,32 @@ __global__ __launch_bounds__(256) void k_dt_partial(const double* __restrict__ U
     }
 }
 
-__global__ __launch_bounds__(256) void k_dt_final(const double* __restrict__ part, const int* __restrict__ part_nan,
-                                                  int n, double* __restrict__ out) {
-    __shared__ double sm[4];
-    __shared__ int sn[4];
+// One block of kDtFinalThreads folds up to tens of thousands of partials: each
+// thread keeps kDtFinalLoads independent loads in flight per round.
+constexpr int kDtFinalThreads = 1024;
+constexpr int kDtFinalLoads = 8;
+
+__global__ __launch_bounds__(kDtFinalThreads) void k_dt_final(const double* __restrict__ part,
+                                                              const int* __restrict__ part_nan, int n,
+                                                              double* __restrict__ out) {
+    __shared__ double sm[kDtFinalThreads / 64];
+    __shared__ int sn[kDtFinalThreads / 64];
     double m = 1.0;
     int nan = 0;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        m = (part[k] < m) ? part[k] : m;
-        nan |= part_nan[k];
+    for (int k0 = threadIdx.x; k0 < n; k0 += kDtFinalThreads * kDtFinalLoads) {
+        double v[kDtFinalLoads];
+        int f[kDtFinalLoads];
+#pragma unroll
+        for (int q = 0; q < kDtFinalLoads; ++q) {
+            const int k = k0 + q * kDtFinalThreads;
+            v[q] = k < n ? part[k] : 1.0;
+            f[q] = k < n ? part_nan[k] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < kDtFinalLoads; ++q) {
+            m = (v[q] < m) ? v[q] : m;
+            nan |= f[q];
+        }
     }
     min_nan_reduce_block(m, nan, sm, sn);
     if (threadIdx.x == 0) {
@@ -389,51 +409,51 @@ hipError_t launch_ghost_fill(double* U, const double* special, const Geom& g, co
 }
 
 template <int S>
-static void launch_fx_s(const double* U, double* Fx, const Geom& g, hipStream_t s) {
-    dim3 grid((g.nx + 1 + kXT - 1) / kXT, g.ny);
-    hipLaunchKernelGGL(k_flux_x<S>, grid, dim3(kXT), 0, s, U, Fx, g);
+static void launch_fy_s(const double* U, double* Fy, const Geom& g, hipStream_t s) {
+    dim3 grid((g.nx + kYC - 1) / kYC, (g.ny + 1 + kYF - 1) / kYF);
+    hipLaunchKernelGGL(k_flux_y<S>, grid, dim3(kYC * kYR), 0, s, U, Fy, g);
 }
 
 template <int S>
-static void launch_fyrk_s(const double* Uin, const double* Fx, const double* Un, double* Uout, double* rhs_out,
+static void launch_fxrk_s(const double* Uin, const double* Fy, const double* Un, double* Uout, double* rhs_out,
                           const Geom& g, const RkCoef& k, double cfl, double* dt_part, int* dt_nan, hipStream_t s) {
-    hipLaunchKernelGGL(k_flux_y_rk<S>, flux_y_rk_grid(g), dim3(kYC * kYR), 0, s, Uin, Fx, Un, Uout, rhs_out, g, k, cfl,
+    hipLaunchKernelGGL(k_flux_x_rk<S>, flux_x_rk_grid(g), dim3(kXT), 0, s, Uin, Fy, Un, Uout, rhs_out, g, k, cfl,
                        dt_part, dt_nan);
 }
 
-hipError_t launch_flux_x(int scheme, const double* U, double* Fx, const Geom& g, hipStream_t s) {
+hipError_t launch_flux_y(int scheme, const double* U, double* Fy, const Geom& g, hipStream_t s) {
     switch (scheme) {
-        case kJS: launch_fx_s<kJS>(U, Fx, g, s); break;
-        case kZ: launch_fx_s<kZ>(U, Fx, g, s); break;
-        case kZP: launch_fx_s<kZP>(U, Fx, g, s); break;
-        default: launch_fx_s<kZPI>(U, Fx, g, s); break;
+        case kJS: launch_fy_s<kJS>(U, Fy, g, s); break;
+        case kZ: launch_fy_s<kZ>(U, Fy, g, s); break;
+        case kZP: launch_fy_s<kZP>(U, Fy, g, s); break;
+        default: launch_fy_s<kZPI>(U, Fy, g, s); break;
     }
     return hipGetLastError();
 }
 
-dim3 flux_y_rk_grid(const Geom& g) { return dim3((g.nx + kYC - 1) / kYC, (g.ny + kYCells - 1) / kYCells); }
+dim3 flux_x_rk_grid(const Geom& g) { return dim3((g.nx + kXC - 1) / kXC, g.ny); }
 
-hipError_t launch_flux_y_rk(int scheme, const double* Uin, const double* Fx, const double* Un, double* Uout,
+hipError_t launch_flux_x_rk(int scheme, const double* Uin, const double* Fy, const double* Un, double* Uout,
                             double* rhs_out, const Geom& g, const RkCoef& k, double cfl, double* dt_part, int* dt_nan,
                             hipStream_t s) {
     switch (scheme) {
-        case kJS: launch_fyrk_s<kJS>(Uin, Fx, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
-        case kZ: launch_fyrk_s<kZ>(Uin, Fx, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
-        case kZP: launch_fyrk_s<kZP>(Uin, Fx, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
-        default: launch_fyrk_s<kZPI>(Uin, Fx, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
+        case kJS: launch_fxrk_s<kJS>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
+        case kZ: launch_fxrk_s<kZ>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
+        case kZP: launch_fxrk_s<kZP>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
+        default: launch_fxrk_s<kZPI>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_dt_final(const double* part, const int* part_nan, int nparts, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, part, part_nan, nparts, out);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kDtFinalThreads), 0, s, part, part_nan, nparts, out);
     return hipGetLastError();
 }
 
 hipError_t launch_dt(const double* U, const Geom& g, double cfl, double* part, int* part_nan, int nblocks,
                      double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_dt_partial, dim3(nblocks), dim3(256), 0, s, U, g, cfl, part, part_nan);
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, part, part_nan, nblocks, out);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kDtFinalThreads), 0, s, part, part_nan, nblocks, out);
     return hipGetLastError();
 }
 

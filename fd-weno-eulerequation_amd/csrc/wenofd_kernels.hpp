@@ -36,13 +36,13 @@ struct RkCoef {
 constexpr int kProbePhysStride = 82;
 
 hipError_t launch_ghost_fill(double* U, const double* special, const Geom& g, const Bc& bc, hipStream_t s);
-hipError_t launch_flux_x(int scheme, const double* U, double* Fx, const Geom& g, hipStream_t s);
-// y faces + rhs + RK stage (see wenofd_kernels.hip). dt_part (optional) receives
-// one partial CFL minimum per block: flux_y_rk_grid(g).x * .y entries.
-hipError_t launch_flux_y_rk(int scheme, const double* Uin, const double* Fx, const double* Un, double* Uout,
+hipError_t launch_flux_y(int scheme, const double* U, double* Fy, const Geom& g, hipStream_t s);
+// x faces + rhs + RK stage (see wenofd_kernels.hip). dt_part (optional) receives
+// one partial CFL minimum per block: flux_x_rk_grid(g).x * .y entries.
+hipError_t launch_flux_x_rk(int scheme, const double* Uin, const double* Fy, const double* Un, double* Uout,
                             double* rhs_out, const Geom& g, const RkCoef& k, double cfl, double* dt_part, int* dt_nan,
                             hipStream_t s);
-dim3 flux_y_rk_grid(const Geom& g);
+dim3 flux_x_rk_grid(const Geom& g);
 hipError_t launch_dt_final(const double* part, const int* part_nan, int nparts, double* out, hipStream_t s);
 hipError_t launch_dt(const double* U, const Geom& g, double cfl, double* part, int* part_nan, int nblocks,
                      double* out, hipStream_t s);

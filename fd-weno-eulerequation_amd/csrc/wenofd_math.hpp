@@ -112,6 +112,67 @@ WENOFD_HD T weno5(T a, T b, T c, T d, T e, double zp_lambda) {
     return (w0 * p0 + w1 * p1) + w2 * p2;
 }
 
+// ---------------------------------------------------------------------------
+// Division without the scaling steps.
+//
+// gfx950 divides in FP64 with: d' = div_scale(d), r = rcp(d'), two Newton
+// refinements r <- fma(r, fma(-d', r, 1), r), n' = div_scale(n), q = n' r,
+// q <- div_fmas(fma(-d', q, n'), r, q), div_fixup(q, d, n). When both operands
+// lie in [2^-300, 2^300] neither div_scale rescales (their exponents differ by
+// less than 768, no operand or quotient is denormal, the numerator is not tiny),
+// div_fmas adds no scale and div_fixup returns q: the sequence reduces to
+// rcp + 4 FMAs that depend on d only, then a mul and two FMAs per numerator,
+// with bit-identical results. The refined reciprocal is therefore shared by
+// divisions with a common denominator. Callers check the range and fall back
+// to the full division outside it.
+// ---------------------------------------------------------------------------
+constexpr double kDivSafeMax = 0x1p300;
+
+__device__ __forceinline__ double rcp_refined(double d) {
+    const double r0 = __builtin_amdgcn_rcp(d);
+    const double r1 = __builtin_fma(r0, __builtin_fma(-d, r0, 1.0), r0);
+    return __builtin_fma(r1, __builtin_fma(-d, r1, 1.0), r1);
+}
+
+__device__ __forceinline__ double div_refined(double n, double d, double r) {
+    const double q = n * r;
+    return __builtin_fma(__builtin_fma(-d, q, n), r, q);
+}
+
+// weno5 with the weight normalisation (three divisions by the common sum s)
+// and WENO-Z's reciprocals 1/(beta + 1e-20) done by div_refined. ok is cleared
+// when an operand leaves the exact range: s >= 1 and 0.1 <= w_k <= s always
+// hold, so s <= 2^300 covers the normalisation; beta + 1e-20 >= 1e-20 always,
+// so an upper bound covers the reciprocals. NaN fails both checks.
+template <int S>
+__device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c, double d, double e,
+                                                   double zp_lambda, bool& ok) {
+    double b0, b1, b2, w0, w1, w2, p0, p1, p2;
+    weno_beta(a, b, c, d, e, b0, b1, b2);
+    if constexpr (S == kZ) {
+        const double eps = 1.0e-20;
+        const double d0 = b0 + eps, d1 = b1 + eps, d2 = b2 + eps;
+        ok = ok && d0 <= kDivSafeMax && d1 <= kDivSafeMax && d2 <= kDivSafeMax;
+        const double i0 = div_refined(1.0, d0, rcp_refined(d0));
+        const double i1 = div_refined(1.0, d1, rcp_refined(d1));
+        const double i2 = div_refined(1.0, d2, rcp_refined(d2));
+        const double tau = fabs(b0 - b2);
+        w0 = 0.1 * (1.0 + (((tau * i0) * tau) * i0));
+        w1 = 0.6 * (1.0 + (((tau * i1) * tau) * i1));
+        w2 = 0.3 * (1.0 + (((tau * i2) * tau) * i2));
+    } else {
+        weno_alpha<S>(b0, b1, b2, zp_lambda, w0, w1, w2);
+    }
+    const double s = (w0 + w1) + w2;
+    ok = ok && s <= kDivSafeMax;
+    const double r = rcp_refined(s);
+    w0 = div_refined(w0, s, r);
+    w1 = div_refined(w1, s, r);
+    w2 = div_refined(w2, s, r);
+    weno_candidates(a, b, c, d, e, p0, p1, p2);
+    return (w0 * p0 + w1 * p1) + w2 * p2;
+}
+
 // Primitive variables with the reference's expression forms
 // (utils/equations.hpp:139-144): p = (g-1)(E - (0.5 rho)(u^2+v^2)), c = sqrt((g p)/rho).
 template <class T>
@@ -253,6 +314,49 @@ WENOFD_HD void face_flux(const T Ul[4], const T Ur[4], GetP sp, GetM sm, double 
         vh[k] = hp + hm;
     }
     project_right<D>(R, vh, Fh);
+}
+
+// face_flux as the kernels run it: the 8 reconstructions use the shared
+// refined reciprocals (weno5_shared_rcp). If an operand of one of them left the
+// exact range in any lane of the wave, the wave recomputes the face with full
+// IEEE divisions and those lanes take that result; the check is wave-uniform
+// and placed after the fast path so that it keeps no extra registers alive.
+template <int D, int S, class GetP, class GetM>
+__device__ __forceinline__ void face_flux_dev(const double Ul[4], const double Ur[4], GetP sp, GetM sm, double gamma,
+                                              double zp_lambda, double Fh[4]) {
+    double ave[4], L[16], R[16];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ave[r] = 0.5 * (Ul[r] + Ur[r]);
+    eigen_matrices<D>(ave, gamma, L, R);
+    double vp[5][4], vm[5][4];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        double fp[4], fm[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            fp[r] = sp(s, r);
+            fm[r] = sm(s + 1, r);
+        }
+        project_left<D>(L, fp, vp[s]);
+        project_left<D>(L, fm, vm[s]);
+    }
+    bool ok = true;
+    double vh[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double hp = weno5_shared_rcp<S>(vp[0][k], vp[1][k], vp[2][k], vp[3][k], vp[4][k], zp_lambda, ok);
+        const double hm = weno5_shared_rcp<S>(vm[4][k], vm[3][k], vm[2][k], vm[1][k], vm[0][k], zp_lambda, ok);
+        vh[k] = hp + hm;
+    }
+    project_right<D>(R, vh, Fh);
+    if (__any(!ok)) {
+        double Fe[4];
+        face_flux<D, S>(Ul, Ur, sp, sm, gamma, zp_lambda, Fe);
+        if (!ok) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Fh[r] = Fe[r];
+        }
+    }
 }
 
 // Local CFL step of one point (src/WENOFD.cpp:419-423, utils/equations.hpp:137-147):

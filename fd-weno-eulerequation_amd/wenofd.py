@@ -10,12 +10,13 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libwenofd_amd.so")
+# WENOFD_LIBRARY selects another build of the same library (A/B timing).
+LIB_PATH = os.environ.get("WENOFD_LIBRARY") or os.path.join(HERE, "lib", "libwenofd_amd.so")
 
 OK, E_CONFIG, E_BAD_ENUM, E_NAN_DT, E_HIP, E_ARG, E_IO = range(7)
 LAYOUT_REF, LAYOUT_SOA, LAYOUT_REF_GHOSTED = 0, 1, 2
 K_GHOST, K_FLUX_X, K_FLUX_Y, K_RK, K_DT, K_COUNT = range(6)
-KERNEL_NAMES = ("ghost_fill", "flux_x", "flux_y_rk", "unused", "delta_t")
+KERNEL_NAMES = ("ghost_fill", "flux_x_rk", "flux_y", "unused", "delta_t")
 
 # Exported symbols declared in include/wenofd_amd.h.
 EXPORTS = (

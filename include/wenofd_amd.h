@@ -47,8 +47,8 @@ enum { WENOFD_LAYOUT_REF = 0, WENOFD_LAYOUT_SOA = 1, WENOFD_LAYOUT_REF_GHOSTED =
 /* Timing kinds for wenofd_get_timing(). */
 enum {
     WENOFD_K_GHOST = 0,   /* ghost fill */
-    WENOFD_K_FLUX_X = 1,  /* x-face fluxes */
-    WENOFD_K_FLUX_Y = 2,  /* y-face fluxes fused with rhs and the RK stage update */
+    WENOFD_K_FLUX_X = 1,  /* x-face fluxes fused with rhs and the RK stage update */
+    WENOFD_K_FLUX_Y = 2,  /* y-face fluxes */
     WENOFD_K_RK = 3,      /* reserved (no separate RK kernel) */
     WENOFD_K_DT = 4,      /* CFL time-step reduction */
     WENOFD_K_COUNT = 5
