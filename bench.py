@@ -59,6 +59,19 @@ def kernel_algorithmic(kind, counts, scheme, nx, ny, rk_first=False):
     return flops, nbytes
 
 
+def measured_traffic(kernel, n):
+    """HBM bytes per launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated), or None when not measured for n."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_vortex2048_wenoz_rk3.md")
+    if n != 2048 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_2048:"):
+            vals = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
+            return int(vals[kernel]) if kernel in vals else None
+    return None
+
+
 def cpu_baseline(nx, ny, scheme, cfl, steps):
     """The CPU oracle (a from-scratch serial restatement of the reference's
     arithmetic) timed on this host over a bounded sample of the same workload."""
@@ -145,7 +158,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": dk["tflops"] / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(dom, n) if args.scheme == "Z" else None,
             "hbm": {"achieved": dk["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dk["gbs"] / HBM_PEAK_GBS},
             "step_fp64_tflops": step_flops / (el / args.steps) / 1e12,
             "kernels": kernels,

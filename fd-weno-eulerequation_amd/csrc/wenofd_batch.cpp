@@ -1,0 +1,526 @@
+// wenofd_batch.cpp — many independent solves advanced together (C ABI
+// wenofd_batch_*): one launch per phase for the whole batch, per-solve time
+// steps kept on the device, the step sequence captured in a HIP graph, tile
+// shapes picked by timing.
+//
+// Each solve is a complete CWENOFD run (src/WENOFD.cpp:189-299) with its own
+// grid, test case, CFL and end time; the batch only shares the scheme and the
+// RK method (template arguments of the kernels). A solve's result equals the
+// single-solve path bit for bit: both run the same tile functions on the same
+// geometry, and the per-solve dt is the same exact minimum.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "wenofd_amd.h"
+#include "wenofd_host.hpp"
+#include "wenofd_kernels.hpp"
+
+using namespace wenofd;
+
+#define BHIP_TRY(expr)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return set_last_error(WENOFD_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct wenofd_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nsolves = 0;
+    int scheme = 0, rkmeth = 2;
+    std::vector<Options> opt;
+    std::vector<CaseDef> cs;
+    std::vector<GridSetup> grid;
+    std::vector<SolveDesc> desc;  // host copy
+    SolveDesc* desc_d = nullptr;
+    SolveState* st_d = nullptr;
+    SolveState* st_h = nullptr;  // pinned
+    SolveState* st_snap = nullptr;
+    double* U[3] = {nullptr, nullptr, nullptr};
+    double* U_snap = nullptr;
+    double* Fy = nullptr;
+    double* special = nullptr;
+    double* part = nullptr;
+    int* part_nan = nullptr;
+    int* active_d = nullptr;
+    int* active_h = nullptr;  // pinned
+    size_t u_total = 0, fy_total = 0, sp_total = 0, part_cap = 0;
+    Tiles tiles{};
+    TileRef* tab_ghost = nullptr;
+    TileRef* tab_y = nullptr;
+    TileRef* tab_x = nullptr;
+    int n_ghost = 0, n_y = 0, n_x = 0;
+    int graph_steps = 16;  // steps per graph replay; 0: plain launches
+    hipGraphExec_t exec = nullptr;
+    double last_ms_per_step = 0;
+
+    BatchView view() const {
+        BatchView v;
+        v.desc = desc_d;
+        v.st = st_d;
+        for (int k = 0; k < 3; ++k) v.U[k] = U[k];
+        v.Fy = Fy;
+        v.special = special;
+        v.part = part;
+        v.part_nan = part_nan;
+        v.active_count = active_d;
+        v.nsolves = nsolves;
+        return v;
+    }
+};
+
+namespace {
+
+int fail(int code, const std::string& msg) { return set_last_error(code, msg); }
+
+void drop_graph(wenofd_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+}
+
+// Block -> (solve, tile) tables of the three per-stage launches for the current
+// tile shape, and the per-solve layout of the dt partials.
+int build_tables(wenofd_batch* b) {
+    std::vector<TileRef> tg, ty, tx;
+    long long part_off = 0;
+    for (int s = 0; s < b->nsolves; ++s) {
+        SolveDesc& d = b->desc[s];
+        const long ng = ghost_blocks(d.g), ny = flux_y_blocks(d.g, b->tiles), nx = flux_x_rk_blocks(d.g, b->tiles);
+        for (long k = 0; k < ng; ++k) tg.push_back({s, (int)k});
+        for (long k = 0; k < ny; ++k) ty.push_back({s, (int)k});
+        for (long k = 0; k < nx; ++k) tx.push_back({s, (int)k});
+        d.part_off = part_off;
+        d.nparts = (int)nx;
+        part_off += nx;
+    }
+    if ((size_t)part_off > b->part_cap) return fail(WENOFD_E_ARG, "dt partial buffer too small for the tile shape");
+    drop_graph(b);
+    (void)hipFree(b->tab_ghost);
+    (void)hipFree(b->tab_y);
+    (void)hipFree(b->tab_x);
+    b->tab_ghost = b->tab_y = b->tab_x = nullptr;
+    BHIP_TRY(hipMalloc(&b->tab_ghost, tg.size() * sizeof(TileRef)));
+    BHIP_TRY(hipMalloc(&b->tab_y, ty.size() * sizeof(TileRef)));
+    BHIP_TRY(hipMalloc(&b->tab_x, tx.size() * sizeof(TileRef)));
+    BHIP_TRY(hipMemcpy(b->tab_ghost, tg.data(), tg.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+    BHIP_TRY(hipMemcpy(b->tab_y, ty.data(), ty.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+    BHIP_TRY(hipMemcpy(b->tab_x, tx.data(), tx.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+    BHIP_TRY(hipMemcpy(b->desc_d, b->desc.data(), b->desc.size() * sizeof(SolveDesc), hipMemcpyHostToDevice));
+    b->n_ghost = (int)tg.size();
+    b->n_y = (int)ty.size();
+    b->n_x = (int)tx.size();
+    return 0;
+}
+
+// The CFL minimum of every solve's current state into its partials, and the
+// count of running solves: the state the first step of a run starts from.
+int prime(wenofd_batch* b) {
+    const BatchView v = b->view();
+    BHIP_TRY(launch_batch_dt_partial(v, b->stream));
+    BHIP_TRY(launch_batch_advance(v, true, b->stream));
+    return 0;
+}
+
+int check_launch(hipError_t e) {
+    return e == hipSuccess ? 0 : fail(WENOFD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+// One stage of every active solve: ghost fill of `in`, y faces, x faces fused
+// with the rhs and the update into `out` (Un = buffer 0).
+int stage(wenofd_batch* b, const BatchView& v, int in, int out, int mode, double a, double bc, double c, bool last) {
+    int rc;
+    if ((rc = check_launch(launch_batch_ghost(v, in, b->tab_ghost, b->n_ghost, b->stream)))) return rc;
+    if ((rc = check_launch(launch_batch_flux_y(b->scheme, b->tiles, v, in, b->tab_y, b->n_y, b->stream)))) return rc;
+    return check_launch(launch_batch_flux_x_rk(b->scheme, b->tiles, v, in, out, mode, a, bc, c, last, b->tab_x,
+                                               b->n_x, b->stream));
+}
+
+// One iteration of the run() loop for every solve (src/WENOFD.cpp:213-239):
+// calculateDeltaT, RunRK<RKMETH>, now += dt. The RK coefficients are those of
+// src/WENOFD.cpp:301-396 (cdt = c * dt formed in the kernel).
+int enqueue_step(wenofd_batch* b) {
+    const BatchView v = b->view();
+    int rc;
+    if ((rc = check_launch(launch_batch_dt(v, b->stream)))) return rc;
+    if (b->rkmeth == 0) {
+        if ((rc = stage(b, v, 0, 1, kRk1, 0.0, 1.0, 1.0, true))) return rc;
+        if ((rc = check_launch(launch_batch_copy(b->tiles, v, 1, 0, b->tab_x, b->n_x, b->stream)))) return rc;
+    } else if (b->rkmeth == 1) {
+        if ((rc = stage(b, v, 0, 1, kRkStage, 1.0, 0.0, 1.0, false))) return rc;
+        if ((rc = stage(b, v, 1, 0, kRkStage, 0.5, 0.5, 0.5, true))) return rc;
+    } else {
+        if ((rc = stage(b, v, 0, 1, kRkStage, 1.0, 0.0, 1.0, false))) return rc;
+        if ((rc = stage(b, v, 1, 2, kRkStage, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, false))) return rc;
+        if ((rc = stage(b, v, 2, 0, kRkStage, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, true))) return rc;
+    }
+    return check_launch(launch_batch_advance(v, false, b->stream));
+}
+
+int ensure_graph(wenofd_batch* b) {
+    if (b->exec || b->graph_steps <= 0) return 0;
+    BHIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int k = 0; k < b->graph_steps && !rc; ++k) rc = enqueue_step(b);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(WENOFD_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiateWithFlags(&b->exec, graph, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        b->exec = nullptr;
+        return fail(WENOFD_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+    }
+    return 0;
+}
+
+// Enqueues n steps: whole graph replays, then single steps for the remainder.
+int enqueue_steps(wenofd_batch* b, int n) {
+    int rc;
+    if (b->graph_steps > 0 && n >= b->graph_steps) {
+        if ((rc = ensure_graph(b))) return rc;
+        for (; n >= b->graph_steps; n -= b->graph_steps) BHIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    }
+    for (; n > 0; --n)
+        if ((rc = enqueue_step(b))) return rc;
+    return 0;
+}
+
+int read_active(wenofd_batch* b, int* active) {
+    BHIP_TRY(hipMemcpyAsync(b->active_h, b->active_d, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    *active = *b->active_h;
+    return 0;
+}
+
+int read_states(wenofd_batch* b) {
+    BHIP_TRY(hipMemcpyAsync(b->st_h, b->st_d, b->nsolves * sizeof(SolveState), hipMemcpyDeviceToHost, b->stream));
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int reset_states(wenofd_batch* b) {
+    for (int s = 0; s < b->nsolves; ++s) b->st_h[s] = SolveState{0.0, 0.0, 0, 0, 0};
+    BHIP_TRY(hipMemcpyAsync(b->st_d, b->st_h, b->nsolves * sizeof(SolveState), hipMemcpyHostToDevice, b->stream));
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int upload_initial(wenofd_batch* b) {
+    for (int s = 0; s < b->nsolves; ++s) {
+        const SolveDesc& d = b->desc[s];
+        std::vector<double> full;
+        initial_state(b->cs[s], b->grid[s], d.g.pitch, d.g.plane, full);
+        for (int k = 0; k < 3; ++k)
+            BHIP_TRY(hipMemcpy(b->U[k] + d.u_off, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    int rc = reset_states(b);
+    return rc ? rc : prime(b);
+}
+
+int create_impl(const char* const* texts, int n, int device, wenofd_batch** out) {
+    auto* b = new wenofd_batch();
+    auto bail = [&](int code) {
+        wenofd_batch_destroy(b);
+        return code;
+    };
+    b->nsolves = n;
+    b->opt.resize(n);
+    b->cs.resize(n);
+    b->grid.resize(n);
+    b->desc.resize(n);
+    auto round_up = [](long long v, long long m) { return (v + m - 1) / m * m; };
+    for (int s = 0; s < n; ++s) {
+        if (!texts[s]) return bail(fail(WENOFD_E_ARG, "null config text"));
+        std::map<std::string, std::string> kv;
+        parse_config_lines(texts[s], kv);
+        std::string err;
+        int rc = interpret_options(kv, b->opt[s], err);
+        if (rc) return bail(fail(rc, "solve " + std::to_string(s) + ": " + err));
+        const Options& o = b->opt[s];
+        if (s == 0) {
+            b->scheme = o.scheme;
+            b->rkmeth = o.rkmeth;
+        } else if (o.scheme != b->scheme || o.rkmeth != b->rkmeth) {
+            return bail(fail(WENOFD_E_CONFIG, "all solves of a batch must share SCHEME and RKMETH"));
+        }
+        make_case(o, b->cs[s]);
+        make_grid(b->cs[s], o.nx, o.ny, b->grid[s]);
+        SolveDesc& d = b->desc[s];
+        Geom& g = d.g;
+        g.nx = o.nx;
+        g.ny = o.ny;
+        g.pitch = (int)round_up(g.nx + 8, 32);
+        g.plane = (long)g.pitch * (g.ny + 8);
+        g.fy_pitch = (int)round_up(g.nx, 32);
+        g.fy_plane = (long)g.fy_pitch * (g.ny + 1);
+        g.dx = b->grid[s].dx;
+        g.dy = b->grid[s].dy;
+        g.gamma = b->cs[s].gamma;
+        g.zp_lambda = b->grid[s].zp_lambda;
+        d.bc = Bc{b->cs[s].bc_left, b->cs[s].bc_right, b->cs[s].bc_bottom};
+        d.gravity = o.testcase == kRTI ? 1 : 0;
+        d.cfl = o.cfl;
+        d.t_end = b->cs[s].t_end;
+        d.u_off = (long long)b->u_total;
+        d.fy_off = (long long)b->fy_total;
+        d.sp_off = (long long)b->sp_total;
+        b->u_total += 4 * (size_t)g.plane;
+        b->fy_total += 4 * (size_t)g.fy_plane;
+        b->sp_total += 4 * (size_t)(2 * (g.nx + g.ny));
+        b->part_cap += (size_t)flux_x_rk_blocks(g, Tiles{64, 64});  // the narrowest x tile
+    }
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return bail(fail(WENOFD_E_HIP, "no HIP device available"));
+    if (device < 0 && hipGetDevice(&device) != hipSuccess) return bail(fail(WENOFD_E_HIP, "hipGetDevice failed"));
+    if (device >= ndev) return bail(fail(WENOFD_E_ARG, "device index out of range"));
+    b->device = device;
+    if (hipSetDevice(device) != hipSuccess) return bail(fail(WENOFD_E_HIP, "hipSetDevice failed"));
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMalloc(&b->U[k], b->u_total * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&b->Fy, b->fy_total * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&b->special, std::max<size_t>(b->sp_total, 1) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&b->part, b->part_cap * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&b->part_nan, b->part_cap * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&b->desc_d, n * sizeof(SolveDesc));
+    if (e == hipSuccess) e = hipMalloc(&b->st_d, n * sizeof(SolveState));
+    if (e == hipSuccess) e = hipMalloc(&b->active_d, sizeof(int));
+    if (e == hipSuccess) e = hipHostMalloc(&b->st_h, n * sizeof(SolveState), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&b->active_h, sizeof(int), hipHostMallocDefault);
+    if (e != hipSuccess) return bail(fail(WENOFD_E_HIP, std::string("allocation: ") + hipGetErrorString(e)));
+    for (int s = 0; s < n; ++s) {
+        std::vector<double> sp;
+        special_band(b->cs[s], b->grid[s], sp);
+        e = hipMemcpy(b->special + b->desc[s].sp_off, sp.data(), sp.size() * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return bail(fail(WENOFD_E_HIP, std::string("upload: ") + hipGetErrorString(e)));
+    }
+    e = hipMemset(b->Fy, 0, b->fy_total * sizeof(double));
+    if (e != hipSuccess) return bail(fail(WENOFD_E_HIP, std::string("memset: ") + hipGetErrorString(e)));
+    int rc;
+    if ((rc = build_tables(b))) return bail(rc);
+    if ((rc = upload_initial(b))) return bail(rc);
+    *out = b;
+    return 0;
+}
+
+int set_tiles_impl(wenofd_batch* b, int x, int y) {
+    Tiles t;
+    t.x = x;
+    t.y = y;
+    if (!tiles_valid(t)) return fail(WENOFD_E_ARG, "tile shape must be x in {64, 128, 256}, y in {16, 32, 64}");
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    b->tiles = t;
+    int rc = build_tables(b);
+    return rc ? rc : prime(b);  // the partials are laid out per x tile
+}
+
+}  // namespace
+
+extern "C" {
+
+int wenofd_batch_create(const char* const* config_texts, int nsolves, int device, wenofd_batch** out) {
+    if (!config_texts || !out || nsolves <= 0) return fail(WENOFD_E_ARG, "bad argument");
+    *out = nullptr;
+    return create_impl(config_texts, nsolves, device, out);
+}
+
+int wenofd_batch_destroy(wenofd_batch* b) {
+    if (!b) return 0;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (double* p : {b->U[0], b->U[1], b->U[2], b->U_snap, b->Fy, b->special, b->part}) (void)hipFree(p);
+    (void)hipFree(b->part_nan);
+    (void)hipFree(b->desc_d);
+    (void)hipFree(b->st_d);
+    (void)hipFree(b->st_snap);
+    (void)hipFree(b->active_d);
+    (void)hipFree(b->tab_ghost);
+    (void)hipFree(b->tab_y);
+    (void)hipFree(b->tab_x);
+    if (b->st_h) (void)hipHostFree(b->st_h);
+    if (b->active_h) (void)hipHostFree(b->active_h);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+int wenofd_batch_size(wenofd_batch* b, int* nsolves) {
+    if (!b || !nsolves) return fail(WENOFD_E_ARG, "null argument");
+    *nsolves = b->nsolves;
+    return 0;
+}
+
+int wenofd_batch_initialize_ave(wenofd_batch* b) {
+    if (!b) return fail(WENOFD_E_ARG, "null handle");
+    BHIP_TRY(hipSetDevice(b->device));
+    return upload_initial(b);
+}
+
+int wenofd_batch_set_tiles(wenofd_batch* b, int x_faces, int y_faces) {
+    if (!b) return fail(WENOFD_E_ARG, "null handle");
+    BHIP_TRY(hipSetDevice(b->device));
+    return set_tiles_impl(b, x_faces, y_faces);
+}
+
+int wenofd_batch_get_tiles(wenofd_batch* b, int* x_faces, int* y_faces) {
+    if (!b || !x_faces || !y_faces) return fail(WENOFD_E_ARG, "null argument");
+    *x_faces = b->tiles.x;
+    *y_faces = b->tiles.y;
+    return 0;
+}
+
+int wenofd_batch_set_graph_steps(wenofd_batch* b, int steps) {
+    if (!b || steps < 0 || steps > 1024) return fail(WENOFD_E_ARG, "graph steps must be in [0, 1024]");
+    BHIP_TRY(hipSetDevice(b->device));
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    drop_graph(b);
+    b->graph_steps = steps;
+    return 0;
+}
+
+int wenofd_batch_advance(wenofd_batch* b, int max_steps, int* steps_done, int* active_left) {
+    if (!b) return fail(WENOFD_E_ARG, "null handle");
+    BHIP_TRY(hipSetDevice(b->device));
+    int rc, active = 0, done = 0;
+    if ((rc = read_active(b, &active))) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (active > 0 && (max_steps <= 0 || done < max_steps)) {
+        int chunk = b->graph_steps > 0 ? b->graph_steps : 64;
+        if (max_steps > 0) chunk = std::min(chunk, max_steps - done);
+        if ((rc = enqueue_steps(b, chunk))) return rc;
+        done += chunk;
+        if ((rc = read_active(b, &active))) return rc;
+    }
+    const double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (done > 0) b->last_ms_per_step = el / done;
+    if (steps_done) *steps_done = done;
+    if (active_left) *active_left = active;
+    if ((rc = read_states(b))) return rc;
+    std::string bad;
+    for (int s = 0; s < b->nsolves; ++s)
+        if (b->st_h[s].nan) bad += (bad.empty() ? "" : ", ") + std::to_string(s);
+    if (!bad.empty()) return fail(WENOFD_E_NAN_DT, "Timestep is not a number (non-physical state) in solve(s) " + bad);
+    return 0;
+}
+
+int wenofd_batch_autotune(wenofd_batch* b, int steps, int* x_faces, int* y_faces, double* ms_per_step) {
+    if (!b || steps <= 0) return fail(WENOFD_E_ARG, "bad argument");
+    BHIP_TRY(hipSetDevice(b->device));
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    if (!b->U_snap) BHIP_TRY(hipMalloc(&b->U_snap, b->u_total * sizeof(double)));
+    if (!b->st_snap) BHIP_TRY(hipMalloc(&b->st_snap, b->nsolves * sizeof(SolveState)));
+    BHIP_TRY(hipMemcpyAsync(b->U_snap, b->U[0], b->u_total * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+    BHIP_TRY(hipMemcpyAsync(b->st_snap, b->st_d, b->nsolves * sizeof(SolveState), hipMemcpyDeviceToDevice, b->stream));
+    auto restore = [&]() -> int {
+        BHIP_TRY(hipMemcpyAsync(b->U[0], b->U_snap, b->u_total * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+        BHIP_TRY(
+            hipMemcpyAsync(b->st_d, b->st_snap, b->nsolves * sizeof(SolveState), hipMemcpyDeviceToDevice, b->stream));
+        return prime(b);
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BHIP_TRY(hipEventCreate(&e0));
+    BHIP_TRY(hipEventCreate(&e1));
+    int rc = 0;
+    float best = 1e30f;
+    Tiles best_t = b->tiles;
+    for (int x : {64, 128, 256}) {
+        for (int y : {16, 32, 64}) {
+            if ((rc = set_tiles_impl(b, x, y))) break;
+            if ((rc = enqueue_steps(b, 1)) || (rc = restore())) break;  // warm-up (first launch of the variant)
+            if ((rc = ensure_graph(b))) break;
+            if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(WENOFD_E_HIP, "event record"); break; }
+            if ((rc = enqueue_steps(b, steps))) break;
+            if (hipEventRecord(e1, b->stream) != hipSuccess) { rc = fail(WENOFD_E_HIP, "event record"); break; }
+            if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(WENOFD_E_HIP, "event sync"); break; }
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = fail(WENOFD_E_HIP, "event time"); break; }
+            if (ms < best) {
+                best = ms;
+                best_t = b->tiles;
+            }
+            if ((rc = restore())) break;
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if ((rc = set_tiles_impl(b, best_t.x, best_t.y))) return rc;
+    if ((rc = restore())) return rc;
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    if (x_faces) *x_faces = best_t.x;
+    if (y_faces) *y_faces = best_t.y;
+    if (ms_per_step) *ms_per_step = best / steps;
+    return 0;
+}
+
+int wenofd_batch_get_state(wenofd_batch* b, int solve, double* host, int layout) {
+    if (!b || !host || solve < 0 || solve >= b->nsolves) return fail(WENOFD_E_ARG, "bad argument");
+    BHIP_TRY(hipSetDevice(b->device));
+    const SolveDesc& d = b->desc[solve];
+    std::vector<double> full(4 * (size_t)d.g.plane);
+    BHIP_TRY(hipMemcpyAsync(full.data(), b->U[0] + d.u_off, full.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            b->stream));
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    if (!unpack_state(full.data(), d.g.nx, d.g.ny, d.g.pitch, d.g.plane, layout, host))
+        return fail(WENOFD_E_ARG, "unknown layout");
+    return 0;
+}
+
+int wenofd_batch_get_info(wenofd_batch* b, int solve, wenofd_info* info) {
+    if (!b || !info || solve < 0 || solve >= b->nsolves) return fail(WENOFD_E_ARG, "bad argument");
+    BHIP_TRY(hipSetDevice(b->device));
+    int rc = read_states(b);
+    if (rc) return rc;
+    const SolveDesc& d = b->desc[solve];
+    const CaseDef& c = b->cs[solve];
+    std::memset(info, 0, sizeof *info);
+    info->cfl = d.cfl;
+    info->dx = d.g.dx;
+    info->dy = d.g.dy;
+    info->xL = c.xL; info->xR = c.xR; info->yL = c.yL; info->yR = c.yR;
+    info->t_end = c.t_end;
+    info->now = b->st_h[solve].now;
+    info->gamma = c.gamma;
+    info->zp_lambda = d.g.zp_lambda;
+    info->steps = b->st_h[solve].steps;
+    info->nx = d.g.nx;
+    info->ny = d.g.ny;
+    info->testcase = b->opt[solve].testcase;
+    info->scheme = b->scheme;
+    info->rkmeth = b->rkmeth;
+    info->device = b->device;
+    return 0;
+}
+
+int wenofd_batch_accuracy(wenofd_batch* b, int solve, double* norms3) {
+    if (!b || !norms3 || solve < 0 || solve >= b->nsolves) return fail(WENOFD_E_ARG, "bad argument");
+    const SolveDesc& d = b->desc[solve];
+    std::vector<double> U((size_t)d.g.nx * d.g.ny * 4);
+    int rc = wenofd_batch_get_state(b, solve, U.data(), WENOFD_LAYOUT_REF);
+    if (rc || (rc = read_states(b))) return rc;
+    rho_error_norms(b->cs[solve], b->grid[solve], b->st_h[solve].now, U, norms3);
+    return 0;
+}
+
+int wenofd_batch_synchronize(wenofd_batch* b) {
+    if (!b) return fail(WENOFD_E_ARG, "null handle");
+    BHIP_TRY(hipSetDevice(b->device));
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -23,10 +23,12 @@ using namespace wenofd;
 
 static thread_local std::string g_last_error;
 
-static int fail(int code, const std::string& msg) {
+int wenofd::set_last_error(int code, const std::string& msg) {
     g_last_error = msg;
     return code;
 }
+
+static int fail(int code, const std::string& msg) { return set_last_error(code, msg); }
 
 #define HIP_TRY(expr)                                                                          \
     do {                                                                                       \
@@ -51,6 +53,7 @@ struct wenofd_solver {
     CaseDef cs;
     Geom g{};
     Bc bc{};
+    Tiles tiles{};
     int device = 0;
     hipStream_t stream = nullptr;
     // Three state buffers (4 planes incl. ghosts each): U is the current
@@ -67,7 +70,7 @@ struct wenofd_solver {
     int dt_parts = 0;           // > 0: the partials describe the current state
     double* dt_dev = nullptr;   // {min, nan flag}
     double* dt_host = nullptr;  // pinned copy of dt_dev
-    std::vector<double> xc, yc;
+    GridSetup grid;
     double now = 0;
     long long steps = 0;
     std::string out_dir;
@@ -140,18 +143,15 @@ int stage(wenofd_solver* h, double* in, const double* un, double* out, const RkC
     int rc;
     if ((rc = timed(h, WENOFD_K_GHOST, [&] { return launch_ghost_fill(in, h->special, h->g, h->bc, h->stream); })))
         return rc;
-    if ((rc = timed(h, WENOFD_K_FLUX_Y, [&] { return launch_flux_y(h->opt.scheme, in, h->Fy, h->g, h->stream); })))
+    if ((rc = timed(h, WENOFD_K_FLUX_Y, [&] { return launch_flux_y(h->opt.scheme, h->tiles, in, h->Fy, h->g, h->stream); })))
         return rc;
     if ((rc = timed(h, WENOFD_K_FLUX_X, [&] {
-             return launch_flux_x_rk(h->opt.scheme, in, h->Fy, un, out, rhs_out, h->g, k, h->opt.cfl,
+             return launch_flux_x_rk(h->opt.scheme, h->tiles, in, h->Fy, un, out, rhs_out, h->g, k, h->opt.cfl,
                                      last_stage ? h->dt_part : nullptr, last_stage ? h->dt_part_nan : nullptr,
                                      h->stream);
          })))
         return rc;
-    if (last_stage) {
-        const dim3 gr = flux_x_rk_grid(h->g);
-        h->dt_parts = (int)(gr.x * gr.y);
-    }
+    if (last_stage) h->dt_parts = (int)flux_x_rk_blocks(h->g, h->tiles);
     return 0;
 }
 
@@ -223,15 +223,8 @@ int download_ref(wenofd_solver* h, std::vector<double>& out) {
 }
 
 int upload_initial(wenofd_solver* h) {
-    // initializeAve (src/WENOFD.cpp:165-188) on the host, interior only.
-    const int nx = h->g.nx, ny = h->g.ny;
-    std::vector<double> full(h->state_doubles(), 0.0);
-    for (int i = 4; i < nx + 4; ++i)
-        for (int j = 4; j < ny + 4; ++j) {
-            double Uc[4];
-            h->cs.conserved0(h->xc[i], h->yc[j], Uc);
-            for (int r = 0; r < 4; ++r) full[r * h->g.plane + (size_t)j * h->g.pitch + i] = Uc[r];
-        }
+    std::vector<double> full;
+    initial_state(h->cs, h->grid, h->g.pitch, h->g.plane, full);
     for (double* b : {h->U, h->Ua, h->Ub})
         HIP_TRY(hipMemcpyAsync(b, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -258,27 +251,19 @@ int create_impl(const std::string& text, int device, wenofd_solver** out) {
     h->opt = o;
     h->device = device;
     make_case(o, h->cs);
+    make_grid(h->cs, o.nx, o.ny, h->grid);
     Geom& g = h->g;
     g.nx = o.nx;
     g.ny = o.ny;
     auto round_up = [](long v, long m) { return (v + m - 1) / m * m; };
     g.pitch = (int)round_up(g.nx + 8, 32);
     g.plane = (long)g.pitch * (g.ny + 8);
-    g.fx_pitch = (int)round_up(g.nx + 1, 32);
-    g.fx_plane = (long)g.fx_pitch * g.ny;
     g.fy_pitch = (int)round_up(g.nx, 32);
     g.fy_plane = (long)g.fy_pitch * (g.ny + 1);
-    // grid conventions, src/WENOFD.cpp:73-106 (one rank: procXLeft = xL + 0 * dx)
-    g.dx = (h->cs.xR - h->cs.xL) / (g.nx - 1);
-    g.dy = (h->cs.yR - h->cs.yL) / (g.ny - 1);
+    g.dx = h->grid.dx;
+    g.dy = h->grid.dy;
     g.gamma = h->cs.gamma;
-    g.zp_lambda = std::pow(g.dx, 2.0 / 3.0);  // include/Utility.hpp:531 with m_deltaX (:836)
-    const double gxl = (h->cs.xL + 0 * g.dx) - g.dx * 4;
-    const double gyl = h->cs.yL - g.dy * 4;
-    h->xc.resize(g.nx + 8);
-    h->yc.resize(g.ny + 8);
-    for (int i = 0; i < g.nx + 8; ++i) h->xc[i] = gxl + i * g.dx;
-    for (int j = 0; j < g.ny + 8; ++j) h->yc[j] = gyl + j * g.dy;
+    g.zp_lambda = h->grid.zp_lambda;
     h->bc = Bc{h->cs.bc_left, h->cs.bc_right, h->cs.bc_bottom};
 
     auto cleanup_fail = [&](int code) {
@@ -293,36 +278,16 @@ int create_impl(const std::string& text, int device, wenofd_solver** out) {
     if (e == hipSuccess) e = hipMalloc(&h->Fy, 4 * (size_t)g.fy_plane * sizeof(double));
     const int ns = 2 * (g.nx + g.ny);
     if (e == hipSuccess) e = hipMalloc(&h->special, 4 * (size_t)ns * sizeof(double));
-    const dim3 fxg = flux_x_rk_grid(g);
-    const size_t nparts = std::max<size_t>(kDtBlocks, (size_t)fxg.x * fxg.y);
+    // room for the partials of the narrowest x tile
+    const size_t nparts = std::max<size_t>(kDtBlocks, (size_t)flux_x_rk_blocks(g, Tiles{64, 64}));
     if (e == hipSuccess) e = hipMalloc(&h->dt_part, nparts * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&h->dt_part_nan, nparts * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&h->dt_dev, 2 * sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc(&h->dt_host, 2 * sizeof(double), hipHostMallocDefault);
     if (e != hipSuccess) return cleanup_fail(fail(WENOFD_E_HIP, std::string("allocation: ") + hipGetErrorString(e)));
 
-    // SPECIAL boundary values (time-invariant): getU0 at the boundary point of
-    // each column / row (src/WENOFD.cpp:1045-1058, 1092-1102, 1138-1149).
-    std::vector<double> sp(4 * (size_t)ns, 0.0);
-    const int sx = 4, ex = g.nx + 4, sy = 4, ey = g.ny + 4;
-    for (int i = 0; i < g.nx; ++i) {
-        double Ub[4], Ut[4];
-        h->cs.conserved0(h->xc[sx + i], h->yc[sy], Ub);
-        h->cs.conserved0(h->xc[sx + i], h->yc[ey - 1], Ut);
-        for (int r = 0; r < 4; ++r) {
-            sp[r * ns + i] = Ub[r];
-            sp[r * ns + g.nx + i] = Ut[r];
-        }
-    }
-    for (int j = 0; j < g.ny; ++j) {
-        double Ul[4], Ur[4];
-        h->cs.conserved0(h->xc[sx], h->yc[sy + j], Ul);
-        h->cs.conserved0(h->xc[ex], h->yc[sy + j], Ur);  // first ghost column's coordinate (:1144)
-        for (int r = 0; r < 4; ++r) {
-            sp[r * ns + 2 * g.nx + j] = Ul[r];
-            sp[r * ns + 2 * g.nx + g.ny + j] = Ur[r];
-        }
-    }
+    std::vector<double> sp;
+    special_band(h->cs, h->grid, sp);
     e = hipMemcpy(h->special, sp.data(), sp.size() * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemsetAsync(h->Fy, 0, 4 * (size_t)g.fy_plane * sizeof(double), h->stream);
     if (e != hipSuccess) return cleanup_fail(fail(WENOFD_E_HIP, std::string("upload: ") + hipGetErrorString(e)));
@@ -364,23 +329,6 @@ int ensure_out_dir(wenofd_solver* h) {
     }
     if (mkdirs(h->out_dir)) return fail(WENOFD_E_IO, "cannot create " + h->out_dir);
     return 0;
-}
-
-void norms(wenofd_solver* h, const std::vector<double>& Uref, double* out3) {
-    // outputAccuracy (src/WENOFD.cpp:1308-1360): loop order x outer, y inner
-    const int nx = h->g.nx, ny = h->g.ny;
-    double e1 = 0, e2 = 0, einf = 0;
-    for (int i = 0; i < nx; ++i)
-        for (int j = 0; j < ny; ++j) {
-            const double ex = h->cs.exact_rho(h->xc[i + 4], h->yc[j + 4], h->now);
-            const double e = std::fabs(ex - Uref[((size_t)i * ny + j) * 4]);
-            e1 += e;
-            e2 += e * e;
-            if (e > einf) einf = e;
-        }
-    out3[0] = einf;
-    out3[1] = e1 / nx / ny;
-    out3[2] = std::sqrt(e2 / nx / ny);
 }
 
 }  // namespace
@@ -496,29 +444,8 @@ static int copy_out(wenofd_solver* h, const double* dev, double* host, int layou
     std::vector<double> full(h->state_doubles());
     HIP_TRY(hipMemcpyAsync(full.data(), dev, full.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    const int nx = h->g.nx, ny = h->g.ny;
-    const Geom& g = h->g;
-    if (layout == WENOFD_LAYOUT_REF) {
-        for (int i = 0; i < nx; ++i)
-            for (int j = 0; j < ny; ++j)
-                for (int r = 0; r < 4; ++r)
-                    host[((size_t)i * ny + j) * 4 + r] = full[r * g.plane + (size_t)(j + 4) * g.pitch + (i + 4)];
-    } else if (layout == WENOFD_LAYOUT_SOA) {
-        for (int r = 0; r < 4; ++r)
-            for (int j = 0; j < ny; ++j)
-                for (int i = 0; i < nx; ++i)
-                    host[((size_t)r * ny + j) * nx + i] = full[r * g.plane + (size_t)(j + 4) * g.pitch + (i + 4)];
-    } else if (layout == WENOFD_LAYOUT_REF_GHOSTED) {
-        for (int i = 0; i < nx + 8; ++i)
-            for (int j = 0; j < ny + 8; ++j) {
-                const bool corner = (i < 4 || i >= nx + 4) && (j < 4 || j >= ny + 4);
-                for (int r = 0; r < 4; ++r)
-                    host[((size_t)i * (ny + 8) + j) * 4 + r] =
-                        corner ? 0.0 : full[r * g.plane + (size_t)j * g.pitch + i];
-            }
-    } else {
+    if (!unpack_state(full.data(), h->g.nx, h->g.ny, h->g.pitch, h->g.plane, layout, host))
         return fail(WENOFD_E_ARG, "unknown layout");
-    }
     return 0;
 }
 
@@ -543,26 +470,8 @@ int wenofd_set_state(wenofd_solver* h, const double* host, int layout) {
     std::vector<double> full(h->state_doubles());
     HIP_TRY(hipMemcpyAsync(full.data(), h->U, full.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    const int nx = h->g.nx, ny = h->g.ny;
-    const Geom& g = h->g;
-    if (layout == WENOFD_LAYOUT_REF) {
-        for (int i = 0; i < nx; ++i)
-            for (int j = 0; j < ny; ++j)
-                for (int r = 0; r < 4; ++r)
-                    full[r * g.plane + (size_t)(j + 4) * g.pitch + (i + 4)] = host[((size_t)i * ny + j) * 4 + r];
-    } else if (layout == WENOFD_LAYOUT_SOA) {
-        for (int r = 0; r < 4; ++r)
-            for (int j = 0; j < ny; ++j)
-                for (int i = 0; i < nx; ++i)
-                    full[r * g.plane + (size_t)(j + 4) * g.pitch + (i + 4)] = host[((size_t)r * ny + j) * nx + i];
-    } else if (layout == WENOFD_LAYOUT_REF_GHOSTED) {
-        for (int i = 0; i < nx + 8; ++i)
-            for (int j = 0; j < ny + 8; ++j)
-                for (int r = 0; r < 4; ++r)
-                    full[r * g.plane + (size_t)j * g.pitch + i] = host[((size_t)i * (ny + 8) + j) * 4 + r];
-    } else {
+    if (!pack_state(host, h->g.nx, h->g.ny, h->g.pitch, h->g.plane, layout, full.data()))
         return fail(WENOFD_E_ARG, "unknown layout");
-    }
     HIP_TRY(hipMemcpyAsync(h->U, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->dt_parts = 0;
@@ -650,7 +559,7 @@ int wenofd_output_ave(wenofd_solver* h, const char* prefix) {
             const double rho = q[0], u = q[1] / q[0], v = q[2] / q[0], E = q[3];
             const double pre = (gm - 1.0) * (E - 0.5 * rho * (u * u + v * v));  // getVitalVarVal, equations.hpp:683-698
             const double c = std::sqrt(gm * pre / rho);
-            std::fprintf(f, "%g %g %g %g %g %g %g %g \n", h->xc[i + 4], h->yc[j + 4], rho, u, v, pre, c, E);
+            std::fprintf(f, "%g %g %g %g %g %g %g %g \n", h->grid.xc[i + 4], h->grid.yc[j + 4], rho, u, v, pre, c, E);
         }
     std::fclose(f);
     return 0;
@@ -670,8 +579,8 @@ int wenofd_output_error(wenofd_solver* h, const char* prefix) {
                  h->g.nx);
     for (int i = 0; i < h->g.nx; ++i)
         for (int j = 0; j < h->g.ny; ++j) {
-            const double ex = h->cs.exact_rho(h->xc[i + 4], h->yc[j + 4], h->now);
-            std::fprintf(f, "%g %g %g \n", h->xc[i + 4], h->yc[j + 4],
+            const double ex = h->cs.exact_rho(h->grid.xc[i + 4], h->grid.yc[j + 4], h->now);
+            std::fprintf(f, "%g %g %g \n", h->grid.xc[i + 4], h->grid.yc[j + 4],
                          std::fabs(ex - U[((size_t)i * h->g.ny + j) * 4]));
         }
     std::fclose(f);
@@ -684,7 +593,7 @@ int wenofd_accuracy(wenofd_solver* h, double* norms3) {
     if (rc) return rc;
     std::vector<double> U;
     if ((rc = download_ref(h, U))) return rc;
-    norms(h, U, norms3);
+    rho_error_norms(h->cs, h->grid, h->now, U, norms3);
     return 0;
 }
 
@@ -751,6 +660,17 @@ int wenofd_get_timing(wenofd_solver* h, double* ms, long long* counts, int nkind
         h->t_ms[k] = 0;
         h->t_cnt[k] = 0;
     }
+    return 0;
+}
+
+int wenofd_set_tiles(wenofd_solver* h, int x_faces, int y_faces) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    Tiles t;
+    t.x = x_faces;
+    t.y = y_faces;
+    if (!tiles_valid(t)) return fail(WENOFD_E_ARG, "tile shape must be x in {64, 128, 256}, y in {16, 32, 64}");
+    h->tiles = t;
+    h->dt_parts = 0;  // the partial minima are laid out per x tile
     return 0;
 }
 

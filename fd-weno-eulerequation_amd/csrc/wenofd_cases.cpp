@@ -266,4 +266,115 @@ void make_case(const Options& o, CaseDef& c) {
     if (o.t_end >= 0) c.t_end = o.t_end;
 }
 
+void make_grid(const CaseDef& c, int nx, int ny, GridSetup& gs) {
+    gs.nx = nx;
+    gs.ny = ny;
+    gs.dx = (c.xR - c.xL) / (nx - 1);
+    gs.dy = (c.yR - c.yL) / (ny - 1);
+    gs.zp_lambda = std::pow(gs.dx, 2.0 / 3.0);  // include/Utility.hpp:531 with m_deltaX (src/WENOFD.cpp:836)
+    const double gxl = (c.xL + 0 * gs.dx) - gs.dx * 4;
+    const double gyl = c.yL - gs.dy * 4;
+    gs.xc.resize(nx + 8);
+    gs.yc.resize(ny + 8);
+    for (int i = 0; i < nx + 8; ++i) gs.xc[i] = gxl + i * gs.dx;
+    for (int j = 0; j < ny + 8; ++j) gs.yc[j] = gyl + j * gs.dy;
+}
+
+void special_band(const CaseDef& c, const GridSetup& gs, std::vector<double>& sp) {
+    const int nx = gs.nx, ny = gs.ny, ns = 2 * (nx + ny);
+    const int sx = 4, ex = nx + 4, sy = 4, ey = ny + 4;
+    sp.assign(4 * (size_t)ns, 0.0);
+    for (int i = 0; i < nx; ++i) {
+        double Ub[4], Ut[4];
+        c.conserved0(gs.xc[sx + i], gs.yc[sy], Ub);
+        c.conserved0(gs.xc[sx + i], gs.yc[ey - 1], Ut);
+        for (int r = 0; r < 4; ++r) {
+            sp[r * ns + i] = Ub[r];
+            sp[r * ns + nx + i] = Ut[r];
+        }
+    }
+    for (int j = 0; j < ny; ++j) {
+        double Ul[4], Ur[4];
+        c.conserved0(gs.xc[sx], gs.yc[sy + j], Ul);
+        c.conserved0(gs.xc[ex], gs.yc[sy + j], Ur);  // first ghost column's coordinate (:1144)
+        for (int r = 0; r < 4; ++r) {
+            sp[r * ns + 2 * nx + j] = Ul[r];
+            sp[r * ns + 2 * nx + ny + j] = Ur[r];
+        }
+    }
+}
+
+void initial_state(const CaseDef& c, const GridSetup& gs, int pitch, long plane, std::vector<double>& full) {
+    full.assign(4 * (size_t)plane, 0.0);
+    for (int i = 4; i < gs.nx + 4; ++i)
+        for (int j = 4; j < gs.ny + 4; ++j) {
+            double Uc[4];
+            c.conserved0(gs.xc[i], gs.yc[j], Uc);
+            for (int r = 0; r < 4; ++r) full[r * plane + (size_t)j * pitch + i] = Uc[r];
+        }
+}
+
+void rho_error_norms(const CaseDef& c, const GridSetup& gs, double t, const std::vector<double>& Uref,
+                     double out3[3]) {
+    const int nx = gs.nx, ny = gs.ny;
+    double e1 = 0, e2 = 0, einf = 0;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            const double ex = c.exact_rho(gs.xc[i + 4], gs.yc[j + 4], t);
+            const double e = std::fabs(ex - Uref[((size_t)i * ny + j) * 4]);
+            e1 += e;
+            e2 += e * e;
+            if (e > einf) einf = e;
+        }
+    out3[0] = einf;
+    out3[1] = e1 / nx / ny;
+    out3[2] = std::sqrt(e2 / nx / ny);
+}
+
+bool unpack_state(const double* full, int nx, int ny, int pitch, long plane, int layout, double* host) {
+    if (layout == WENOFD_LAYOUT_REF) {
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j)
+                for (int r = 0; r < 4; ++r)
+                    host[((size_t)i * ny + j) * 4 + r] = full[r * plane + (size_t)(j + 4) * pitch + (i + 4)];
+    } else if (layout == WENOFD_LAYOUT_SOA) {
+        for (int r = 0; r < 4; ++r)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                    host[((size_t)r * ny + j) * nx + i] = full[r * plane + (size_t)(j + 4) * pitch + (i + 4)];
+    } else if (layout == WENOFD_LAYOUT_REF_GHOSTED) {
+        for (int i = 0; i < nx + 8; ++i)
+            for (int j = 0; j < ny + 8; ++j) {
+                const bool corner = (i < 4 || i >= nx + 4) && (j < 4 || j >= ny + 4);
+                for (int r = 0; r < 4; ++r)
+                    host[((size_t)i * (ny + 8) + j) * 4 + r] = corner ? 0.0 : full[r * plane + (size_t)j * pitch + i];
+            }
+    } else {
+        return false;
+    }
+    return true;
+}
+
+bool pack_state(const double* host, int nx, int ny, int pitch, long plane, int layout, double* full) {
+    if (layout == WENOFD_LAYOUT_REF) {
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j)
+                for (int r = 0; r < 4; ++r)
+                    full[r * plane + (size_t)(j + 4) * pitch + (i + 4)] = host[((size_t)i * ny + j) * 4 + r];
+    } else if (layout == WENOFD_LAYOUT_SOA) {
+        for (int r = 0; r < 4; ++r)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                    full[r * plane + (size_t)(j + 4) * pitch + (i + 4)] = host[((size_t)r * ny + j) * nx + i];
+    } else if (layout == WENOFD_LAYOUT_REF_GHOSTED) {
+        for (int i = 0; i < nx + 8; ++i)
+            for (int j = 0; j < ny + 8; ++j)
+                for (int r = 0; r < 4; ++r)
+                    full[r * plane + (size_t)j * pitch + i] = host[((size_t)i * (ny + 8) + j) * 4 + r];
+    } else {
+        return false;
+    }
+    return true;
+}
+
 }  // namespace wenofd

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <map>
 #include <string>
+#include <vector>
 
 namespace wenofd {
 
@@ -55,6 +56,39 @@ void make_case(const Options& o, CaseDef& c);
 // splitmix64 stream -> 53-bit uniforms; per quadrant (UR, UL, LL, LR) rho ~
 // U[0.1, 2], u ~ U[-1.5, 1.5], v ~ U[-1.5, 1.5], p ~ U[0.05, 2], in that order.
 void seeded_quadrant_states(uint64_t seed, double out[4][4]);
+
+// Point coordinates of one solve including 4 ghost points per side, grid
+// conventions of src/WENOFD.cpp:73-106 for one rank (procXLeft = xL).
+struct GridSetup {
+    int nx = 0, ny = 0;
+    double dx = 0, dy = 0, zp_lambda = 0;
+    std::vector<double> xc, yc;  // nx + 8 and ny + 8 entries
+};
+void make_grid(const CaseDef& c, int nx, int ny, GridSetup& gs);
+
+// SPECIAL boundary values (time-invariant), [4][2 nx + 2 ny]: bottom row,
+// top row, left column, right column (src/WENOFD.cpp:1045-1058, 1092-1102,
+// 1138-1149).
+void special_band(const CaseDef& c, const GridSetup& gs, std::vector<double>& sp);
+
+// initializeAve (src/WENOFD.cpp:165-188) into a zeroed 4-plane buffer with the
+// given row pitch and plane size (interior points only).
+void initial_state(const CaseDef& c, const GridSetup& gs, int pitch, long plane, std::vector<double>& full);
+
+// {Linf, L1, L2} of rho against the exact solution at time t (outputAccuracy,
+// src/WENOFD.cpp:1308-1360: x outer, y inner). Uref in the [nx][ny][4] order.
+void rho_error_norms(const CaseDef& c, const GridSetup& gs, double t, const std::vector<double>& Uref,
+                     double out3[3]);
+
+// Conversions between a device-layout state buffer (4 planes, row pitch,
+// ghosts) and the C ABI's host layouts (WENOFD_LAYOUT_*). Return false for an
+// unknown layout.
+bool unpack_state(const double* full, int nx, int ny, int pitch, long plane, int layout, double* host);
+bool pack_state(const double* host, int nx, int ny, int pitch, long plane, int layout, double* full);
+
+// Records msg as the calling thread's last error (wenofd_last_error) and
+// returns code.
+int set_last_error(int code, const std::string& msg);
 
 extern const char* const kCaseNames[6];
 extern const char* const kSchemeNames[4];

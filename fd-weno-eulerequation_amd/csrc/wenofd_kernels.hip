@@ -9,6 +9,10 @@
 //
 // One RK stage = ghost fill -> y faces -> x faces fused with rhs + RK update.
 // All arithmetic is FP64 in the reference's operation order (wenofd_math.hpp).
+//
+// Every phase is a tile function shared by two kernels: one solve per launch
+// (blockIdx = tile), and a batch of independent solves per launch (a table
+// maps each block to a solve and a tile of it, so uneven grids share a launch).
 
 #include <hip/hip_runtime.h>
 
@@ -23,11 +27,12 @@ namespace wenofd {
 // sweeps read (x ghosts of interior rows, y ghosts of interior columns; corner
 // ghosts are never read). Pure copies and sign flips: bitwise.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_ghost_fill(double* __restrict__ U, const double* __restrict__ special,
-                                                    Geom g, Bc bc) {
+constexpr int kGhostThreads = 256;
+
+__device__ __forceinline__ void ghost_point(double* __restrict__ U, const double* __restrict__ special,
+                                            const Geom& g, const Bc& bc, int k) {
     const int nxg = 8 * g.ny;  // x-ghost points: 4 left + 4 right per interior row
     const int nyg = 8 * g.nx;  // y-ghost points
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nxg + nyg) return;
     const int sx = 4, ex = g.nx + 4, sy = 4, ey = g.ny + 4;
     int di, dj, si, sj, flip = -1, spec = -1;
@@ -82,29 +87,35 @@ __global__ __launch_bounds__(256) void k_ghost_fill(double* __restrict__ U, cons
     }
 }
 
-// ---------------------------------------------------------------------------
-// y faces. One block = 64 interior columns x 64 faces along y, marched in
-// steps of 4 face rows (256 threads = 64 columns x 4 rows). The LLF split of
-// the 9 point rows one step reads lives in an LDS ring; each step adds 4 new
-// rows, so every point is split once per block (plus the 5-row lead-in).
-// Loads of U are coalesced along x.
-// ---------------------------------------------------------------------------
-constexpr int kYC = 64;    // columns per block
-constexpr int kYR = 4;     // face rows per step
-constexpr int kYF = 64;    // faces per block along y
-constexpr int kRing = 9;   // ring rows: a step reads 9 consecutive point rows
+__global__ __launch_bounds__(kGhostThreads) void k_ghost_fill(double* __restrict__ U,
+                                                              const double* __restrict__ special, Geom g, Bc bc) {
+    ghost_point(U, special, g, bc, blockIdx.x * kGhostThreads + threadIdx.x);
+}
 
-template <int S>
-__global__ __launch_bounds__(kYC* kYR) void k_flux_y(const double* __restrict__ U, double* __restrict__ Fy, Geom g) {
+// ---------------------------------------------------------------------------
+// y faces. One tile = 64 interior columns x YF faces along y, marched in steps
+// of 4 face rows (256 threads = 64 columns x 4 rows). The LLF split of the 9
+// point rows one step reads lives in an LDS ring; each step adds 4 new rows, so
+// every point is split once per tile (plus the 5-row lead-in). Loads of U are
+// coalesced along x.
+// ---------------------------------------------------------------------------
+constexpr int kYC = 64;   // columns per tile
+constexpr int kYR = 4;    // face rows per step
+constexpr int kRing = 9;  // ring rows: a step reads 9 consecutive point rows
+
+template <int S, int YF>
+__device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double* __restrict__ Fy, const Geom& g,
+                                            int bx, int by) {
+    static_assert(YF % kYR == 0, "YF must be a multiple of the step height");
     __shared__ double sP[4][kRing][kYC];
     __shared__ double sM[4][kRing][kYC];
     const int tx = threadIdx.x % kYC;
     const int ty = threadIdx.x / kYC;
-    const int i = 4 + blockIdx.x * kYC + tx;
+    const int i = 4 + bx * kYC + tx;
     const bool col_ok = i < g.nx + 4;
-    const int F0 = 4 + blockIdx.y * kYF;     // first face of the block
-    const int F1 = min(F0 + kYF, g.ny + 5);  // one past the last face
-    const int base = F0 - 3;                 // point row held in ring slot 0
+    const int F0 = 4 + by * YF;             // first face of the tile
+    const int F1 = min(F0 + YF, g.ny + 5);  // one past the last face
+    const int base = F0 - 3;                // point row held in ring slot 0
 
     auto load_row = [&](int prow) {
         const int slot = (prow - base) % kRing;
@@ -146,48 +157,57 @@ __global__ __launch_bounds__(kYC* kYR) void k_flux_y(const double* __restrict__ 
     }
 }
 
+template <int YF>
+__host__ __device__ inline int y_tiles_x(const Geom& g) { return (g.nx + kYC - 1) / kYC; }
+template <int YF>
+__host__ __device__ inline int y_tiles_y(const Geom& g) { return (g.ny + 1 + YF - 1) / YF; }
+
+template <int S, int YF>
+__global__ __launch_bounds__(kYC* kYR) void k_flux_y(const double* __restrict__ U, double* __restrict__ Fy, Geom g) {
+    flux_y_tile<S, YF>(U, Fy, g, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // x faces fused with the rhs and the RK stage update.
 //
-// One block = 256 consecutive faces of one interior row = the 255 cells
-// between them (consecutive blocks overlap by one face). The LLF split of the
-// 261 points the faces read is computed once into LDS (component-major: a
-// wave's 64 consecutive slots hit 64 distinct banks), then each thread builds
-// one face flux into LDS and completes one cell:
+// One tile = XT consecutive faces of one interior row = the XT-1 cells between
+// them (consecutive tiles overlap by one face). The LLF split of the XT+5
+// points the faces read is computed once into LDS (component-major: a wave's
+// 64 consecutive slots hit 64 distinct banks), then each thread builds one face
+// flux into LDS and completes one cell:
 //   rhs = x term + y term (faces read from Fy) [+ RTI source], then
 //   kRkNone:  rhs only (rhs_out)           kRk1:   out = in + dt rhs
 //   kRkStage: out = (a Un + b in) + (c dt) rhs
-// `out` is never `in` (neighbouring blocks still read `in`); it may be `Un`,
-// which is read and written only at the thread's own cell. With dt_part set,
-// the block also reduces the local CFL step of its updated cells (the next
-// calculateDeltaT) into dt_part[block] / dt_nan[block].
+// `out` is never `in` (neighbouring tiles still read `in`); it may be `Un`,
+// which is read and written only at the thread's own cell. With part set, the
+// tile also reduces the local CFL step of its updated cells (the next
+// calculateDeltaT) into *part / *part_nan.
 // ---------------------------------------------------------------------------
-constexpr int kXT = 256;      // faces per block
-constexpr int kXC = kXT - 1;  // cells per block
-constexpr int kXP = kXT + 5;  // points per block
-constexpr int kXS = kXT + 8;  // LDS row stride (slots)
-
-template <int S>
-__global__ __launch_bounds__(kXT) void k_flux_x_rk(const double* __restrict__ Uin, const double* __restrict__ Fy,
-                                                   const double* Un, double* Uout, double* __restrict__ rhs_out,
-                                                   Geom g, RkCoef k, double cfl, double* __restrict__ dt_part,
-                                                   int* __restrict__ dt_nan) {
-    __shared__ double sP[4][kXS];
-    __shared__ double sM[4][kXS];
-    __shared__ double sU[4][kXS];
-    __shared__ double sF[4][kXT];
-    __shared__ double sMin[kXT / 64];
-    __shared__ int sNan[kXT / 64];
+template <int S, int XT>
+__device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, const double* __restrict__ Fy,
+                                               const double* Un, double* Uout, double* __restrict__ rhs_out,
+                                               const Geom& g, const RkCoef& k, double cfl, double* part,
+                                               int* part_nan, int bx, int row) {
+    constexpr int XC = XT - 1;  // cells per tile
+    constexpr int XP = XT + 5;  // points per tile
+    constexpr int XS = XT + 8;  // LDS row stride (slots)
+    constexpr int W = XT / 64;  // waves per tile
+    __shared__ double sP[4][XS];
+    __shared__ double sM[4][XS];
+    __shared__ double sU[4][XS];
+    __shared__ double sF[4][XT];
+    __shared__ double sMin[W];
+    __shared__ int sNan[W];
     const int t = threadIdx.x;
-    const int j = blockIdx.y + 4;
-    const int f0 = 4 + blockIdx.x * kXC;  // first face = first cell of the block
-    const double* row = Uin + (size_t)j * g.pitch;
-    for (int q = t; q < kXP; q += kXT) {
+    const int j = row + 4;
+    const int f0 = 4 + bx * XC;  // first face = first cell of the tile
+    const double* rowp = Uin + (size_t)j * g.pitch;
+    for (int q = t; q < XP; q += XT) {
         const int p = f0 - 3 + q;  // point p sits in slot p - f0 + 3
         if (p < g.nx + 8) {
             double u[4], fm[4], fp[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) u[r] = row[r * g.plane + p];
+            for (int r = 0; r < 4; ++r) u[r] = rowp[r * g.plane + p];
             flux_split<0>(u, g.gamma, fm, fp);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -213,7 +233,7 @@ __global__ __launch_bounds__(kXT) void k_flux_x_rk(const double* __restrict__ Ui
         for (int r = 0; r < 4; ++r) sF[r][t] = Fh[r];
     }
     const int i = f0 + t;  // this thread's cell
-    const bool cell_ok = t < kXC && i < g.nx + 4;
+    const bool cell_ok = t < XC && i < g.nx + 4;
     const size_t o = (size_t)j * g.pitch + i;
     const size_t oy = (size_t)(j - 4) * g.fy_pitch + (i - 4);
     double fyb[4], fyt[4], unb[4];
@@ -250,14 +270,14 @@ __global__ __launch_bounds__(kXT) void k_flux_x_rk(const double* __restrict__ Ui
                 nu[r] = (k.mode == kRk1) ? u[r] + k.cdt * rhs[r] : rk_combine(k.a, unb[r], k.b, u[r], k.cdt, rhs[r]);
 #pragma unroll
             for (int r = 0; r < 4; ++r) Uout[r * g.plane + o] = nu[r];
-            if (dt_part) {
+            if (part) {
                 const double tl = local_dt(nu, g.gamma, cfl, g.dx, g.dy);
                 if (tl != tl) my_nan = 1;
                 my_min = (tl < my_min) ? tl : my_min;
             }
         }
     }
-    if (dt_part) {
+    if (part) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             const double ov = __shfl_xor(my_min, off, 64);
@@ -270,24 +290,37 @@ __global__ __launch_bounds__(kXT) void k_flux_x_rk(const double* __restrict__ Ui
         }
         __syncthreads();
         if (t == 0) {
-            for (int q = 1; q < kXT / 64; ++q) {
+            for (int q = 1; q < W; ++q) {
                 my_min = (sMin[q] < my_min) ? sMin[q] : my_min;
                 my_nan |= sNan[q];
             }
-            const int b = blockIdx.y * gridDim.x + blockIdx.x;
-            dt_part[b] = my_min;
-            dt_nan[b] = my_nan;
+            *part = my_min;
+            *part_nan = my_nan;
         }
     }
+}
+
+template <int XT>
+__host__ __device__ inline int x_tiles_x(const Geom& g) { return (g.nx + (XT - 1) - 1) / (XT - 1); }
+
+template <int S, int XT>
+__global__ __launch_bounds__(XT) void k_flux_x_rk(const double* __restrict__ Uin, const double* __restrict__ Fy,
+                                                  const double* Un, double* Uout, double* __restrict__ rhs_out,
+                                                  Geom g, RkCoef k, double cfl, double* __restrict__ dt_part,
+                                                  int* __restrict__ dt_nan) {
+    const long b = (long)blockIdx.y * gridDim.x + blockIdx.x;
+    flux_x_rk_tile<S, XT>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part ? dt_part + b : nullptr,
+                          dt_part ? dt_nan + b : nullptr, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
 // calculateDeltaT, src/WENOFD.cpp:398-452 with utils/equations.hpp:137-147:
 // per interior point cfl / ((|u|+c)/dx + (|v|+c)/dy), minimum over the domain.
-// Two passes: per-block minimum (wavefront shuffles, then LDS), then one block
-// folds the partials. MIN is exact, so the result does not depend on order.
-// A NaN anywhere raises the flag (the reference aborts on a NaN step).
+// Partial minima per block (wavefront shuffles, then LDS), folded by one block.
+// MIN is exact, so the result does not depend on the order. A NaN anywhere
+// raises the flag (the reference aborts on a NaN step).
 // ---------------------------------------------------------------------------
+template <int NW>
 __device__ __forceinline__ void min_nan_reduce_block(double& m, int& nan, double* sm, int* sn) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -296,18 +329,28 @@ __device__ __forceinline__ void min_nan_reduce_block(double& m, int& nan, double
         m = (o < m) ? o : m;
         nan |= on;
     }
-    const int lane = threadIdx.x % 64, w = threadIdx.x / 64, nw = blockDim.x / 64;
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
     if (lane == 0) {
         sm[w] = m;
         sn[w] = nan;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int q = 1; q < nw; ++q) {
+        for (int q = 1; q < NW; ++q) {
             m = (sm[q] < m) ? sm[q] : m;
             nan |= sn[q];
         }
     }
+}
+
+__device__ __forceinline__ void dt_point(const double* __restrict__ U, const Geom& g, double cfl, long k, double& m,
+                                         int& nan) {
+    const int jl = (int)(k / g.nx), il = (int)(k % g.nx);
+    const size_t o = (size_t)(jl + 4) * g.pitch + (il + 4);
+    const double u[4] = {U[o], U[g.plane + o], U[2 * g.plane + o], U[3 * g.plane + o]};
+    const double tmp = local_dt(u, g.gamma, cfl, g.dx, g.dy);
+    if (tmp != tmp) nan = 1;
+    m = (tmp < m) ? tmp : m;
 }
 
 __global__ __launch_bounds__(256) void k_dt_partial(const double* __restrict__ U, Geom g, double cfl,
@@ -317,39 +360,32 @@ __global__ __launch_bounds__(256) void k_dt_partial(const double* __restrict__ U
     double m = 1.0;
     int nan = 0;
     const long total = (long)g.nx * g.ny;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (long)gridDim.x * blockDim.x) {
-        const int jl = (int)(k / g.nx), il = (int)(k % g.nx);
-        const size_t o = (size_t)(jl + 4) * g.pitch + (il + 4);
-        const double u[4] = {U[o], U[g.plane + o], U[2 * g.plane + o], U[3 * g.plane + o]};
-        const double tmp = local_dt(u, g.gamma, cfl, g.dx, g.dy);
-        if (tmp != tmp) nan = 1;
-        m = (tmp < m) ? tmp : m;
-    }
-    min_nan_reduce_block(m, nan, sm, sn);
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (long)gridDim.x * blockDim.x)
+        dt_point(U, g, cfl, k, m, nan);
+    min_nan_reduce_block<4>(m, nan, sm, sn);
     if (threadIdx.x == 0) {
         part[blockIdx.x] = m;
         part_nan[blockIdx.x] = nan;
     }
 }
 
-// One block of kDtFinalThreads folds up to tens of thousands of partials: each
-// thread keeps kDtFinalLoads independent loads in flight per round.
+// One block folds up to hundreds of thousands of partials: each thread keeps
+// kDtFinalLoads independent loads in flight per round.
 constexpr int kDtFinalThreads = 1024;
 constexpr int kDtFinalLoads = 8;
 
-__global__ __launch_bounds__(kDtFinalThreads) void k_dt_final(const double* __restrict__ part,
-                                                              const int* __restrict__ part_nan, int n,
-                                                              double* __restrict__ out) {
+__device__ __forceinline__ void fold_partials(const double* __restrict__ part, const int* __restrict__ part_nan,
+                                              long n, double& m, int& nan) {
     __shared__ double sm[kDtFinalThreads / 64];
     __shared__ int sn[kDtFinalThreads / 64];
-    double m = 1.0;
-    int nan = 0;
-    for (int k0 = threadIdx.x; k0 < n; k0 += kDtFinalThreads * kDtFinalLoads) {
+    m = 1.0;
+    nan = 0;
+    for (long k0 = threadIdx.x; k0 < n; k0 += (long)kDtFinalThreads * kDtFinalLoads) {
         double v[kDtFinalLoads];
         int f[kDtFinalLoads];
 #pragma unroll
         for (int q = 0; q < kDtFinalLoads; ++q) {
-            const int k = k0 + q * kDtFinalThreads;
+            const long k = k0 + (long)q * kDtFinalThreads;
             v[q] = k < n ? part[k] : 1.0;
             f[q] = k < n ? part_nan[k] : 0;
         }
@@ -359,11 +395,143 @@ __global__ __launch_bounds__(kDtFinalThreads) void k_dt_final(const double* __re
             nan |= f[q];
         }
     }
-    min_nan_reduce_block(m, nan, sm, sn);
+    min_nan_reduce_block<kDtFinalThreads / 64>(m, nan, sm, sn);
+}
+
+__global__ __launch_bounds__(kDtFinalThreads) void k_dt_final(const double* __restrict__ part,
+                                                              const int* __restrict__ part_nan, int n,
+                                                              double* __restrict__ out) {
+    double m;
+    int nan;
+    fold_partials(part, part_nan, n, m, nan);
     if (threadIdx.x == 0) {
         out[0] = m;
         out[1] = nan ? 1.0 : 0.0;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Batched phases. Block b works on tile tiles[b].tile of solve tiles[b].solve;
+// solves that are not active in this step return at once.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kGhostThreads) void k_batch_ghost(BatchView b, int buf, const TileRef* __restrict__ tiles) {
+    const TileRef tr = tiles[blockIdx.x];
+    if (!b.st[tr.solve].active) return;
+    const SolveDesc& d = b.desc[tr.solve];
+    ghost_point(b.U[buf] + d.u_off, b.special + d.sp_off, d.g, d.bc, tr.tile * kGhostThreads + threadIdx.x);
+}
+
+template <int S, int YF>
+__global__ __launch_bounds__(kYC* kYR) void k_batch_flux_y(BatchView b, int buf, const TileRef* __restrict__ tiles) {
+    const TileRef tr = tiles[blockIdx.x];
+    if (!b.st[tr.solve].active) return;
+    const SolveDesc& d = b.desc[tr.solve];
+    const int ntx = y_tiles_x<YF>(d.g);
+    flux_y_tile<S, YF>(b.U[buf] + d.u_off, b.Fy + d.fy_off, d.g, tr.tile % ntx, tr.tile / ntx);
+}
+
+template <int S, int XT>
+__global__ __launch_bounds__(XT) void k_batch_flux_x_rk(BatchView b, int in, int out, int mode, double a,
+                                                        double bcoef, double c_stage, int last,
+                                                        const TileRef* __restrict__ tiles) {
+    const TileRef tr = tiles[blockIdx.x];
+    const SolveState& st = b.st[tr.solve];
+    if (!st.active) return;
+    const SolveDesc& d = b.desc[tr.solve];
+    RkCoef k{mode, d.gravity, a, bcoef, c_stage * st.dt};
+    const int ntx = x_tiles_x<XT>(d.g);
+    double* part = last ? b.part + d.part_off + tr.tile : nullptr;
+    int* pnan = last ? b.part_nan + d.part_off + tr.tile : nullptr;
+    flux_x_rk_tile<S, XT>(b.U[in] + d.u_off, b.Fy + d.fy_off, b.U[0] + d.u_off, b.U[out] + d.u_off, nullptr, d.g, k,
+                          d.cfl, part, pnan, tr.tile % ntx, tr.tile / ntx);
+}
+
+// First step: each solve's CFL minimum straight from its state, into the first
+// of its partial slots (the rest are set neutral).
+__global__ __launch_bounds__(kDtFinalThreads) void k_batch_dt_partial(BatchView b) {
+    __shared__ double sm[kDtFinalThreads / 64];
+    __shared__ int sn[kDtFinalThreads / 64];
+    const SolveDesc& d = b.desc[blockIdx.x];
+    const double* U = b.U[0] + d.u_off;
+    double m = 1.0;
+    int nan = 0;
+    const long total = (long)d.g.nx * d.g.ny;
+    for (long k = threadIdx.x; k < total; k += kDtFinalThreads) dt_point(U, d.g, d.cfl, k, m, nan);
+    min_nan_reduce_block<kDtFinalThreads / 64>(m, nan, sm, sn);
+    for (int q = threadIdx.x; q < d.nparts; q += kDtFinalThreads) {
+        b.part[d.part_off + q] = 1.0;
+        b.part_nan[d.part_off + q] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        b.part[d.part_off] = m;
+        b.part_nan[d.part_off] = nan;
+    }
+}
+
+// Per solve (one block each): the loop test |now - t_end| > 1e-9
+// (src/WENOFD.cpp:213), then dt from the partials, clipped to t_end - now
+// (:434-435); a NaN step stops the solve.
+__global__ __launch_bounds__(kDtFinalThreads) void k_batch_dt(BatchView b) {
+    SolveState& st = b.st[blockIdx.x];
+    const SolveDesc& d = b.desc[blockIdx.x];
+    const bool run = !st.nan && fabs(st.now - d.t_end) > 1e-9;
+    if (!run) {
+        if (threadIdx.x == 0) {
+            st.active = 0;
+            st.dt = 0.0;
+        }
+        return;
+    }
+    double m;
+    int nan;
+    fold_partials(b.part + d.part_off, b.part_nan + d.part_off, d.nparts, m, nan);
+    if (threadIdx.x == 0) {
+        if (nan) {
+            st.nan = 1;
+            st.active = 0;
+            st.dt = 0.0;
+        } else {
+            double ts = m;
+            if (d.t_end - st.now < ts) ts = d.t_end - st.now;
+            st.dt = ts;
+            st.active = 1;
+        }
+    }
+}
+
+// now += dt for the solves this step advanced (unless count_only); then the
+// number of solves the next step will advance.
+__global__ __launch_bounds__(256) void k_batch_advance(BatchView b, int count_only) {
+    __shared__ int sc[4];
+    int cnt = 0;
+    for (int s = threadIdx.x; s < b.nsolves; s += 256) {
+        SolveState& st = b.st[s];
+        if (st.active && !count_only) {
+            st.now += st.dt;
+            st.steps += 1;
+        }
+        if (!st.nan && fabs(st.now - b.desc[s].t_end) > 1e-9) ++cnt;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    if (threadIdx.x % 64 == 0) sc[threadIdx.x / 64] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) *b.active_count = sc[0] + sc[1] + sc[2] + sc[3];
+}
+
+template <int XT>
+__global__ __launch_bounds__(XT) void k_batch_copy(BatchView b, int from, int to, const TileRef* __restrict__ tiles) {
+    const TileRef tr = tiles[blockIdx.x];
+    if (!b.st[tr.solve].active) return;
+    const SolveDesc& d = b.desc[tr.solve];
+    const int ntx = x_tiles_x<XT>(d.g);
+    const int i = 4 + (tr.tile % ntx) * (XT - 1) + threadIdx.x;
+    const int j = 4 + tr.tile / ntx;
+    if (threadIdx.x >= XT - 1 || i >= d.g.nx + 4) return;
+    const size_t o = d.u_off + (size_t)j * d.g.pitch + i;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) b.U[to][o + r * d.g.plane] = b.U[from][o + r * d.g.plane];
 }
 
 // ---------------------------------------------------------------------------
@@ -375,7 +543,9 @@ __global__ void k_probe_weno(const double* __restrict__ v5, int n, double zp_lam
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const double* v = v5 + 5 * (size_t)k;
-    out[k] = weno5<S>(v[0], v[1], v[2], v[3], v[4], zp_lambda);
+    bool ok = true;
+    const double fast = weno5_shared_rcp<S>(v[0], v[1], v[2], v[3], v[4], zp_lambda, ok);
+    out[k] = ok ? fast : weno5<S>(v[0], v[1], v[2], v[3], v[4], zp_lambda);
 }
 
 // per state: Lx(16) Rx(16) Ly(16) Ry(16) Fm_x Fp_x Fm_y Fp_y (4 each) maxeig_x maxeig_y
@@ -400,49 +570,75 @@ __global__ void k_probe_physics(const double* __restrict__ Uin, int n, double ga
 }
 
 // ---------------------------------------------------------------------------
-// Launch wrappers.
+// Launch wrappers: scheme and tile shape become template arguments here.
 // ---------------------------------------------------------------------------
+bool tiles_valid(const Tiles& t) {
+    return (t.x == 64 || t.x == 128 || t.x == 256) && (t.y == 16 || t.y == 32 || t.y == 64);
+}
+
+template <class F>
+static hipError_t with_scheme(int scheme, F&& f) {
+    switch (scheme) {
+        case kJS: f(std::integral_constant<int, kJS>()); break;
+        case kZ: f(std::integral_constant<int, kZ>()); break;
+        case kZP: f(std::integral_constant<int, kZP>()); break;
+        default: f(std::integral_constant<int, kZPI>()); break;
+    }
+    return hipGetLastError();
+}
+
+template <class F>
+static void with_x_tile(int xt, F&& f) {
+    if (xt == 64) f(std::integral_constant<int, 64>());
+    else if (xt == 128) f(std::integral_constant<int, 128>());
+    else f(std::integral_constant<int, 256>());
+}
+
+template <class F>
+static void with_y_tile(int yf, F&& f) {
+    if (yf == 16) f(std::integral_constant<int, 16>());
+    else if (yf == 32) f(std::integral_constant<int, 32>());
+    else f(std::integral_constant<int, 64>());
+}
+
+long ghost_blocks(const Geom& g) { return (8L * (g.nx + g.ny) + kGhostThreads - 1) / kGhostThreads; }
+
+long flux_y_blocks(const Geom& g, const Tiles& t) {
+    long n = 0;
+    with_y_tile(t.y, [&](auto YF) { n = (long)y_tiles_x<YF()>(g) * y_tiles_y<YF()>(g); });
+    return n;
+}
+
+long flux_x_rk_blocks(const Geom& g, const Tiles& t) {
+    long n = 0;
+    with_x_tile(t.x, [&](auto XT) { n = (long)x_tiles_x<XT()>(g) * g.ny; });
+    return n;
+}
+
 hipError_t launch_ghost_fill(double* U, const double* special, const Geom& g, const Bc& bc, hipStream_t s) {
-    const int n = 8 * (g.nx + g.ny);
-    hipLaunchKernelGGL(k_ghost_fill, dim3((n + 255) / 256), dim3(256), 0, s, U, special, g, bc);
+    hipLaunchKernelGGL(k_ghost_fill, dim3(ghost_blocks(g)), dim3(kGhostThreads), 0, s, U, special, g, bc);
     return hipGetLastError();
 }
 
-template <int S>
-static void launch_fy_s(const double* U, double* Fy, const Geom& g, hipStream_t s) {
-    dim3 grid((g.nx + kYC - 1) / kYC, (g.ny + 1 + kYF - 1) / kYF);
-    hipLaunchKernelGGL(k_flux_y<S>, grid, dim3(kYC * kYR), 0, s, U, Fy, g);
+hipError_t launch_flux_y(int scheme, const Tiles& t, const double* U, double* Fy, const Geom& g, hipStream_t s) {
+    return with_scheme(scheme, [&](auto S) {
+        with_y_tile(t.y, [&](auto YF) {
+            dim3 grid(y_tiles_x<YF()>(g), y_tiles_y<YF()>(g));
+            hipLaunchKernelGGL((k_flux_y<S(), YF()>), grid, dim3(kYC * kYR), 0, s, U, Fy, g);
+        });
+    });
 }
 
-template <int S>
-static void launch_fxrk_s(const double* Uin, const double* Fy, const double* Un, double* Uout, double* rhs_out,
-                          const Geom& g, const RkCoef& k, double cfl, double* dt_part, int* dt_nan, hipStream_t s) {
-    hipLaunchKernelGGL(k_flux_x_rk<S>, flux_x_rk_grid(g), dim3(kXT), 0, s, Uin, Fy, Un, Uout, rhs_out, g, k, cfl,
-                       dt_part, dt_nan);
-}
-
-hipError_t launch_flux_y(int scheme, const double* U, double* Fy, const Geom& g, hipStream_t s) {
-    switch (scheme) {
-        case kJS: launch_fy_s<kJS>(U, Fy, g, s); break;
-        case kZ: launch_fy_s<kZ>(U, Fy, g, s); break;
-        case kZP: launch_fy_s<kZP>(U, Fy, g, s); break;
-        default: launch_fy_s<kZPI>(U, Fy, g, s); break;
-    }
-    return hipGetLastError();
-}
-
-dim3 flux_x_rk_grid(const Geom& g) { return dim3((g.nx + kXC - 1) / kXC, g.ny); }
-
-hipError_t launch_flux_x_rk(int scheme, const double* Uin, const double* Fy, const double* Un, double* Uout,
-                            double* rhs_out, const Geom& g, const RkCoef& k, double cfl, double* dt_part, int* dt_nan,
-                            hipStream_t s) {
-    switch (scheme) {
-        case kJS: launch_fxrk_s<kJS>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
-        case kZ: launch_fxrk_s<kZ>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
-        case kZP: launch_fxrk_s<kZP>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
-        default: launch_fxrk_s<kZPI>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part, dt_nan, s); break;
-    }
-    return hipGetLastError();
+hipError_t launch_flux_x_rk(int scheme, const Tiles& t, const double* Uin, const double* Fy, const double* Un,
+                            double* Uout, double* rhs_out, const Geom& g, const RkCoef& k, double cfl,
+                            double* dt_part, int* dt_nan, hipStream_t s) {
+    return with_scheme(scheme, [&](auto S) {
+        with_x_tile(t.x, [&](auto XT) {
+            dim3 grid(x_tiles_x<XT()>(g), g.ny);
+            hipLaunchKernelGGL((k_flux_x_rk<S(), XT()>), grid, dim3(XT()), 0, s, Uin, Fy, Un, Uout, rhs_out, g, k,
+                               cfl, dt_part, dt_nan);
+        });
+    });
 }
 
 hipError_t launch_dt_final(const double* part, const int* part_nan, int nparts, double* out, hipStream_t s) {
@@ -458,18 +654,61 @@ hipError_t launch_dt(const double* U, const Geom& g, double cfl, double* part, i
 }
 
 hipError_t launch_probe_weno(int scheme, const double* v5, int n, double zp_lambda, double* out, hipStream_t s) {
-    dim3 grid((n + 255) / 256), block(256);
-    switch (scheme) {
-        case kJS: hipLaunchKernelGGL(k_probe_weno<kJS>, grid, block, 0, s, v5, n, zp_lambda, out); break;
-        case kZ: hipLaunchKernelGGL(k_probe_weno<kZ>, grid, block, 0, s, v5, n, zp_lambda, out); break;
-        case kZP: hipLaunchKernelGGL(k_probe_weno<kZP>, grid, block, 0, s, v5, n, zp_lambda, out); break;
-        default: hipLaunchKernelGGL(k_probe_weno<kZPI>, grid, block, 0, s, v5, n, zp_lambda, out); break;
-    }
-    return hipGetLastError();
+    return with_scheme(scheme, [&](auto S) {
+        hipLaunchKernelGGL(k_probe_weno<S()>, dim3((n + 255) / 256), dim3(256), 0, s, v5, n, zp_lambda, out);
+    });
 }
 
 hipError_t launch_probe_physics(const double* U, int n, double gamma, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_probe_physics, dim3((n + 127) / 128), dim3(128), 0, s, U, n, gamma, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_ghost(const BatchView& b, int buf, const TileRef* tiles, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_ghost, dim3(nblocks), dim3(kGhostThreads), 0, s, b, buf, tiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_flux_y(int scheme, const Tiles& t, const BatchView& b, int buf, const TileRef* tiles,
+                               int nblocks, hipStream_t s) {
+    return with_scheme(scheme, [&](auto S) {
+        with_y_tile(t.y, [&](auto YF) {
+            hipLaunchKernelGGL((k_batch_flux_y<S(), YF()>), dim3(nblocks), dim3(kYC * kYR), 0, s, b, buf, tiles);
+        });
+    });
+}
+
+hipError_t launch_batch_flux_x_rk(int scheme, const Tiles& t, const BatchView& b, int in, int out, int mode,
+                                  double a, double bcoef, double c_stage, bool last, const TileRef* tiles,
+                                  int nblocks, hipStream_t s) {
+    return with_scheme(scheme, [&](auto S) {
+        with_x_tile(t.x, [&](auto XT) {
+            hipLaunchKernelGGL((k_batch_flux_x_rk<S(), XT()>), dim3(nblocks), dim3(XT()), 0, s, b, in, out, mode, a,
+                               bcoef, c_stage, last ? 1 : 0, tiles);
+        });
+    });
+}
+
+hipError_t launch_batch_dt_partial(const BatchView& b, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_dt_partial, dim3(b.nsolves), dim3(kDtFinalThreads), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_dt(const BatchView& b, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_dt, dim3(b.nsolves), dim3(kDtFinalThreads), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_advance(const BatchView& b, bool count_only, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(256), 0, s, b, count_only ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_copy(const Tiles& t, const BatchView& b, int from, int to, const TileRef* tiles, int nblocks,
+                             hipStream_t s) {
+    with_x_tile(t.x, [&](auto XT) {
+        hipLaunchKernelGGL(k_batch_copy<XT()>, dim3(nblocks), dim3(XT()), 0, s, b, from, to, tiles);
+    });
     return hipGetLastError();
 }
 

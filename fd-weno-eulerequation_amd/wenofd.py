@@ -26,7 +26,11 @@ EXPORTS = (
     "wenofd_get_state", "wenofd_set_state", "wenofd_get_info", "wenofd_set_time", "wenofd_output_dir",
     "wenofd_output_ave", "wenofd_output_error", "wenofd_output_accuracy", "wenofd_accuracy",
     "wenofd_set_timing", "wenofd_get_timing", "wenofd_synchronize", "wenofd_probe_weno5",
-    "wenofd_probe_physics",
+    "wenofd_probe_physics", "wenofd_set_tiles",
+    "wenofd_batch_create", "wenofd_batch_destroy", "wenofd_batch_size", "wenofd_batch_initialize_ave",
+    "wenofd_batch_advance", "wenofd_batch_set_graph_steps", "wenofd_batch_set_tiles", "wenofd_batch_get_tiles",
+    "wenofd_batch_autotune", "wenofd_batch_get_state", "wenofd_batch_get_info", "wenofd_batch_accuracy",
+    "wenofd_batch_synchronize",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -82,6 +86,20 @@ def lib():
         L.wenofd_get_timing.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong), i]
         L.wenofd_probe_weno5.argtypes = [i, i, d, i, _dp, _dp]
         L.wenofd_probe_physics.argtypes = [i, d, i, _dp, _dp]
+        L.wenofd_set_tiles.argtypes = [vp, i, i]
+        ip = ctypes.POINTER(i)
+        L.wenofd_batch_create.argtypes = [ctypes.POINTER(ctypes.c_char_p), i, i, ctypes.POINTER(vp)]
+        for n in ("wenofd_batch_destroy", "wenofd_batch_initialize_ave", "wenofd_batch_synchronize"):
+            getattr(L, n).argtypes = [vp]
+        L.wenofd_batch_size.argtypes = [vp, ip]
+        L.wenofd_batch_advance.argtypes = [vp, i, ip, ip]
+        L.wenofd_batch_set_graph_steps.argtypes = [vp, i]
+        L.wenofd_batch_set_tiles.argtypes = [vp, i, i]
+        L.wenofd_batch_get_tiles.argtypes = [vp, ip, ip]
+        L.wenofd_batch_autotune.argtypes = [vp, i, ip, ip, _dp]
+        L.wenofd_batch_get_state.argtypes = [vp, i, _dp, i]
+        L.wenofd_batch_get_info.argtypes = [vp, i, ctypes.POINTER(Info)]
+        L.wenofd_batch_accuracy.argtypes = [vp, i, _dp]
         _LIB = L
     return _LIB
 
@@ -116,6 +134,23 @@ def probe_physics(states, gamma=1.4, device=0):
     s = np.ascontiguousarray(states, dtype=np.float64).reshape(-1, 4)
     out = np.empty((s.shape[0], 82))
     _check(lib().wenofd_probe_physics(device, float(gamma), s.shape[0], _ptr(s), _ptr(out)))
+    return out
+
+
+def vortex_batch_configs(n=64, seed=0, scheme=1, rkmeth=2, cfl=0.05, sizes=range(100, 401, 20), t_end=None):
+    """Config texts of n seeded isentropic-vortex solves: N drawn uniformly
+    from `sizes` (square grids), strength ~ U[3, 6], centre ~ U[3, 7]^2."""
+    rng = np.random.default_rng(seed)
+    sizes = list(sizes)
+    out = []
+    for _ in range(n):
+        N = int(sizes[rng.integers(len(sizes))])
+        keys = dict(TESTCASE=4, SCHEME=scheme, RKMETH=rkmeth, CFL="%.17g" % cfl, ELEMNUMX=N, ELEMNUMY=N,
+                    VORTEX_EPS="%.17g" % rng.uniform(3.0, 6.0), VORTEX_XC="%.17g" % rng.uniform(3.0, 7.0),
+                    VORTEX_YC="%.17g" % rng.uniform(3.0, 7.0))
+        if t_end is not None:
+            keys["TEND"] = "%.17g" % t_end
+        out.append(config_text(**keys))
     return out
 
 
@@ -239,5 +274,84 @@ class Solver:
         _check(lib().wenofd_get_timing(self.h, _ptr(ms), cnt, K_COUNT))
         return {KERNEL_NAMES[k]: (float(ms[k]), int(cnt[k])) for k in range(K_COUNT)}
 
+    def set_tiles(self, x_faces, y_faces):
+        _check(lib().wenofd_set_tiles(self.h, int(x_faces), int(y_faces)))
+
     def synchronize(self):
         _check(lib().wenofd_synchronize(self.h))
+
+
+class Batch:
+    """Many independent solves advanced together (wenofd_batch handle)."""
+
+    def __init__(self, configs, device=0):
+        texts = [c.encode() if isinstance(c, str) else config_text(**c).encode() for c in configs]
+        arr = (ctypes.c_char_p * len(texts))(*texts)
+        h = ctypes.c_void_p()
+        _check(lib().wenofd_batch_create(arr, len(texts), device, ctypes.byref(h)))
+        self.h = h
+        self.n = len(texts)
+        self.shapes = [(inf.nx, inf.ny) for inf in (self.info(s) for s in range(self.n))]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().wenofd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self, solve):
+        inf = Info()
+        _check(lib().wenofd_batch_get_info(self.h, int(solve), ctypes.byref(inf)))
+        return inf
+
+    def initialize_ave(self):
+        _check(lib().wenofd_batch_initialize_ave(self.h))
+
+    def advance(self, max_steps=0):
+        """Returns (iterations launched, solves still running)."""
+        done, active = ctypes.c_int(), ctypes.c_int()
+        _check(lib().wenofd_batch_advance(self.h, int(max_steps), ctypes.byref(done), ctypes.byref(active)))
+        return done.value, active.value
+
+    def set_graph_steps(self, steps):
+        _check(lib().wenofd_batch_set_graph_steps(self.h, int(steps)))
+
+    def set_tiles(self, x_faces, y_faces):
+        _check(lib().wenofd_batch_set_tiles(self.h, int(x_faces), int(y_faces)))
+
+    def get_tiles(self):
+        x, y = ctypes.c_int(), ctypes.c_int()
+        _check(lib().wenofd_batch_get_tiles(self.h, ctypes.byref(x), ctypes.byref(y)))
+        return x.value, y.value
+
+    def autotune(self, steps=4):
+        """Returns (x_faces, y_faces, ms per step of the chosen shape)."""
+        x, y, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().wenofd_batch_autotune(self.h, int(steps), ctypes.byref(x), ctypes.byref(y), ctypes.byref(ms)))
+        return x.value, y.value, ms.value
+
+    def get_state(self, solve, layout=LAYOUT_REF):
+        nx, ny = self.shapes[solve]
+        shape = {LAYOUT_REF: (nx, ny, 4), LAYOUT_SOA: (4, ny, nx)}.get(layout, (nx + 8, ny + 8, 4))
+        out = np.empty(shape)
+        _check(lib().wenofd_batch_get_state(self.h, int(solve), _ptr(out), layout))
+        return out
+
+    def accuracy(self, solve):
+        out = np.empty(3)
+        _check(lib().wenofd_batch_accuracy(self.h, int(solve), _ptr(out)))
+        return out
+
+    def synchronize(self):
+        _check(lib().wenofd_batch_synchronize(self.h))

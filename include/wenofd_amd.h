@@ -140,6 +140,50 @@ int wenofd_get_timing(wenofd_solver* h, double* ms, long long* counts, int nkind
 /* Blocks until the solver's stream is idle. */
 int wenofd_synchronize(wenofd_solver* h);
 
+/* Tile shapes of the two face sweeps: x_faces faces per block of the fused
+ * x-face/rhs/RK kernel (64, 128 or 256), y_faces faces per block of the y-face
+ * kernel (16, 32 or 64). Results are bitwise independent of the shape. No
+ * counterpart in the reference (a launch-shape knob of this implementation). */
+int wenofd_set_tiles(wenofd_solver* h, int x_faces, int y_faces);
+
+/*
+ * Batched solves. No counterpart in the reference, which runs one case per
+ * process: a batch holds many independent CWENOFD runs (each its own config
+ * text: grid size, test case, CFL, TEND, vortex parameters ...; all sharing
+ * SCHEME and RKMETH) and advances them together, one kernel launch per phase
+ * for the whole batch. Each solve's time step is computed and applied on the
+ * device (calculateDeltaT, src/WENOFD.cpp:398-452, clipped to its own t_end);
+ * a solve that reaches its t_end (|now - t_end| <= 1e-9, :213) or meets a NaN
+ * step stops while the others go on. The step sequence is captured once in a
+ * HIP graph of graph_steps steps and replayed. Each solve's state equals the
+ * single-solve path (wenofd_advance) bit for bit.
+ */
+typedef struct wenofd_batch wenofd_batch;
+
+int wenofd_batch_create(const char* const* config_texts, int nsolves, int device, wenofd_batch** out);
+int wenofd_batch_destroy(wenofd_batch* b);
+int wenofd_batch_size(wenofd_batch* b, int* nsolves);
+/* Every solve back to its initial condition at time 0. */
+int wenofd_batch_initialize_ave(wenofd_batch* b);
+/* Advances every running solve by up to max_steps run-loop iterations
+ * (max_steps <= 0: until all reach t_end). steps_done = iterations launched;
+ * active_left = solves still running. WENOFD_E_NAN_DT if a solve met a NaN
+ * step (it stopped; wenofd_last_error names it). */
+int wenofd_batch_advance(wenofd_batch* b, int max_steps, int* steps_done, int* active_left);
+/* Steps per graph replay (default 16; 0 launches each kernel directly). */
+int wenofd_batch_set_graph_steps(wenofd_batch* b, int steps);
+/* Tile shapes (as wenofd_set_tiles) for every solve of the batch. */
+int wenofd_batch_set_tiles(wenofd_batch* b, int x_faces, int y_faces);
+int wenofd_batch_get_tiles(wenofd_batch* b, int* x_faces, int* y_faces);
+/* Times `steps` steps of the batch under each tile shape, starting each trial
+ * from the current state, keeps the fastest and restores the state. */
+int wenofd_batch_autotune(wenofd_batch* b, int steps, int* x_faces, int* y_faces, double* ms_per_step);
+int wenofd_batch_get_state(wenofd_batch* b, int solve, double* host, int layout);
+int wenofd_batch_get_info(wenofd_batch* b, int solve, wenofd_info* info);
+/* {Linf, L1, L2} of rho of one solve against its exact solution. */
+int wenofd_batch_accuracy(wenofd_batch* b, int solve, double* norms3);
+int wenofd_batch_synchronize(wenofd_batch* b);
+
 /* Probes of the device arithmetic on independent inputs (host arrays):
  * WENO5 reconstruction (include/Utility.hpp:415-615) of n 5-tuples, and per
  * state {Lx, Rx, Ly, Ry (16 each, row-major), Fm_x, Fp_x, Fm_y, Fp_y (4

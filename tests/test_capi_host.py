@@ -92,6 +92,29 @@ def test_create_errors_before_touching_a_device():
     assert rc == W.E_CONFIG
 
 
+def test_batch_config_errors_before_touching_a_device():
+    """A batch shares SCHEME and RKMETH; a bad member config names its index."""
+    a = W.config_text(TESTCASE=4, SCHEME=1, RKMETH=2, CFL=0.5, ELEMNUMX=16, ELEMNUMY=16)
+    b = W.config_text(TESTCASE=4, SCHEME=0, RKMETH=2, CFL=0.5, ELEMNUMX=16, ELEMNUMY=16)
+    bad = W.config_text(TESTCASE=4, SCHEME=1, RKMETH=2, CFL=0.5, ELEMNUMX=3, ELEMNUMY=16)
+    h = ctypes.c_void_p()
+    for texts, code, text in (([a, b], W.E_CONFIG, "share SCHEME"), ([a, bad], W.E_CONFIG, "solve 1")):
+        arr = (ctypes.c_char_p * len(texts))(*[t.encode() for t in texts])
+        rc = W.lib().wenofd_batch_create(arr, len(texts), 0, ctypes.byref(h))
+        assert rc == code and not h.value
+        assert text in W.lib().wenofd_last_error().decode()
+    assert W.lib().wenofd_batch_create(None, 0, 0, ctypes.byref(h)) == W.E_ARG
+
+
+def test_vortex_batch_configs_cover_the_size_range():
+    cfgs = W.vortex_batch_configs(n=64, seed=7)
+    infos = [W.parse_config(c) for c in cfgs]
+    sizes = {i.nx for i in infos}
+    assert len(cfgs) == 64 and sizes <= set(range(100, 401, 20)) and len(sizes) > 8
+    assert all(i.nx == i.ny and i.testcase == 4 and i.scheme == 1 and i.rkmeth == 2 for i in infos)
+    assert cfgs == W.vortex_batch_configs(n=64, seed=7)  # seeded
+
+
 def test_no_silent_fallback_without_gpu():
     """Without a usable GPU the library refuses to run rather than computing on the CPU."""
     if os.path.exists("/dev/kfd"):
