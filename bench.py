@@ -88,6 +88,45 @@ def cpu_baseline(nx, ny, scheme, cfl, steps):
                       f"{[k for k, v in SCHEMES.items() if v == scheme][0]}, serial C++ oracle, {el:.1f} s"}
 
 
+def batched(W, steps, seq_steps, seed=0):
+    """64 seeded vortex solves (N in {100, 120, ..., 400}, strength ~ U[3, 6],
+    centre ~ U[3, 7]^2), WENO-Z RK3 CFL 0.05, advanced as one batch (one
+    launch per phase, graph-captured steps, autotuned tiles) against the same
+    64 solves run one after another through the single-solve path."""
+    cfgs = W.vortex_batch_configs(n=64, seed=seed, cfl=0.05)
+    cells = sum(W.parse_config(c).nx * W.parse_config(c).ny for c in cfgs)
+    log(f"batch: 64 vortex solves, {cells} cells; autotune")
+    b = W.Batch(cfgs)
+    x, y, _ = b.autotune(steps=8)
+    b.advance(16)
+    b.synchronize()
+    t0 = time.perf_counter()
+    done, active = b.advance(steps)
+    b.synchronize()
+    el = time.perf_counter() - t0
+    assert done == steps and active == 64
+    b.close()
+    log(f"batch: {el * 1e3 / steps:.3f} ms/step, tiles {x}x{y}; sequential {seq_steps} steps each")
+    solvers = [W.Solver(c) for c in cfgs]
+    for s in solvers:
+        s.advance(2)
+        s.synchronize()
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.advance(seq_steps)
+    for s in solvers:
+        s.synchronize()
+    el_seq = time.perf_counter() - t0
+    for s in solvers:
+        s.close()
+    value, seq = cells * steps / el, cells * seq_steps / el_seq
+    return {"workload": "64 isentropic-vortex solves, N seeded-uniform in {100,120,...,400}, strength ~U[3,6], "
+                        "centre ~U[3,7]^2, WENO-Z RK3 CFL 0.05, per-solve dt on the device",
+            "cells": cells, "steps": steps, "value": value, "unit": "cell-updates/s",
+            "ms_per_step": el * 1e3 / steps, "tiles": [x, y], "graph_steps": 16,
+            "sequential_value": seq, "sequential_steps": seq_steps, "speedup_vs_sequential": value / seq}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +136,8 @@ def main():
     ap.add_argument("--scheme", default="Z", choices=list(SCHEMES))
     ap.add_argument("--cfl", type=float, default=0.6)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for cpu_baseline (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=200, help="steps of the 64-solve batch (0: skip)")
+    ap.add_argument("--seq-steps", type=int, default=20, help="steps per solve of the sequential comparison")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this solver runs one whole domain on one GPU; --gpus must be 1", file=sys.stderr)
@@ -164,6 +205,8 @@ def main():
             "kernels": kernels,
         },
     }
+    if args.batch_steps > 0:
+        rec["batched"] = batched(W, args.batch_steps, args.seq_steps)
     if args.cpu_steps > 0:
         log(f"cpu baseline: {args.cpu_steps} oracle steps at {n}x{n}")
         rec["cpu_baseline"] = cpu_baseline(n, n, sc, args.cfl, args.cpu_steps)
