@@ -18,6 +18,8 @@
 #include "wenofd_amd.h"
 #include "wenofd_host.hpp"
 #include "wenofd_kernels.hpp"
+#include "wenofd_mlp.hpp"
+#include "wenofd_switch.hpp"
 
 using namespace wenofd;
 
@@ -79,8 +81,25 @@ struct wenofd_solver {
     std::vector<hipEvent_t> event_pool;
     double t_ms[WENOFD_K_COUNT] = {};
     long long t_cnt[WENOFD_K_COUNT] = {};
+    // Learned switch: mode, MLP parameters (device), threshold, raw and
+    // dilated flags per direction (x faces [ny][nx+1], y faces [ny+1][nx]).
+    int sw_mode = WENOFD_SWITCH_OFF;
+    float sw_thr = 0.0f;
+    float* sw_params = nullptr;
+    unsigned char* fl_raw_x = nullptr;
+    unsigned char* fl_raw_y = nullptr;
+    unsigned char* fl_x = nullptr;
+    unsigned char* fl_y = nullptr;
+    // After a switched step the state before it survives in *prev_slot (one of
+    // Ua / Ub) until the next step; the plain solver redoes the step from it
+    // when the switched step produced a NaN time step.
+    double** prev_slot = nullptr;
+    double prev_dt = 0;
+    long long fallbacks = 0;
 
     size_t state_doubles() const { return 4 * (size_t)g.plane; }
+    long faces_x() const { return (long)g.ny * (g.nx + 1); }
+    long faces_y() const { return (long)(g.ny + 1) * g.nx; }
 };
 
 namespace {
@@ -135,30 +154,90 @@ int set_dev(wenofd_solver* h) {
 }
 
 
-// One RK stage: ghost fill of `in`, y-face fluxes, then x faces fused with the
-// rhs and the update into `out` (which also reduces the next CFL step when
-// last_stage is set).
+// One RK stage: ghost fill of `in` (unless done), y-face fluxes, then x faces
+// fused with the rhs and the update into `out` (which also reduces the next
+// CFL step when last_stage is set). With the switch on, faces use the flags.
 int stage(wenofd_solver* h, double* in, const double* un, double* out, const RkCoef& k, double* rhs_out,
-          bool last_stage) {
+          bool last_stage, bool ghost = true) {
     int rc;
-    if ((rc = timed(h, WENOFD_K_GHOST, [&] { return launch_ghost_fill(in, h->special, h->g, h->bc, h->stream); })))
+    const bool sw = h->sw_mode != WENOFD_SWITCH_OFF;
+    if (ghost &&
+        (rc = timed(h, WENOFD_K_GHOST, [&] { return launch_ghost_fill(in, h->special, h->g, h->bc, h->stream); })))
         return rc;
-    if ((rc = timed(h, WENOFD_K_FLUX_Y, [&] { return launch_flux_y(h->opt.scheme, h->tiles, in, h->Fy, h->g, h->stream); })))
+    if ((rc = timed(h, WENOFD_K_FLUX_Y, [&] {
+             return launch_flux_y(h->opt.scheme, h->tiles, in, h->Fy, h->g, h->stream, sw ? h->fl_y : nullptr);
+         })))
         return rc;
     if ((rc = timed(h, WENOFD_K_FLUX_X, [&] {
              return launch_flux_x_rk(h->opt.scheme, h->tiles, in, h->Fy, un, out, rhs_out, h->g, k, h->opt.cfl,
                                      last_stage ? h->dt_part : nullptr, last_stage ? h->dt_part_nan : nullptr,
-                                     h->stream);
+                                     h->stream, sw ? h->fl_x : nullptr);
          })))
         return rc;
     if (last_stage) h->dt_parts = (int)flux_x_rk_blocks(h->g, h->tiles);
     return 0;
 }
 
-int run_rk(wenofd_solver* h, int rkmeth, double dt) {
+// Ghost fill of U, then (MLP mode) the flags of every face from the network,
+// dilated by one face in both directions of the face grid: the flags of a
+// step are computed once from its initial state and serve all its stages.
+int classify(wenofd_solver* h, float* logits_x = nullptr, float* logits_y = nullptr, bool dilate = true) {
+    int rc;
+    if ((rc = timed(h, WENOFD_K_GHOST, [&] { return launch_ghost_fill(h->U, h->special, h->g, h->bc, h->stream); })))
+        return rc;
+    if (h->sw_mode != WENOFD_SWITCH_MLP) return 0;
+    return timed(h, WENOFD_K_SWITCH, [&] {
+        hipError_t e = launch_classify(0, h->U, h->g, h->sw_params, h->sw_thr, h->fl_raw_x, logits_x, h->stream);
+        if (e == hipSuccess)
+            e = launch_classify(1, h->U, h->g, h->sw_params, h->sw_thr, h->fl_raw_y, logits_y, h->stream);
+        if (e == hipSuccess && dilate) e = launch_dilate(h->fl_raw_x, h->fl_x, h->g.ny, h->g.nx + 1, h->stream);
+        if (e == hipSuccess && dilate) e = launch_dilate(h->fl_raw_y, h->fl_y, h->g.ny + 1, h->g.nx, h->stream);
+        return e;
+    });
+}
+
+// RK methods with the switch on: flags from the step's initial state, and the
+// final stage written outside U so that the initial state survives the step.
+int run_rk_switched(wenofd_solver* h, int rkmeth, double dt) {
     const int grav = h->opt.testcase == kRTI ? 1 : 0;
     int rc;
     h->dt_parts = 0;
+    h->prev_slot = nullptr;
+    if ((rc = classify(h))) return rc;
+    if (rkmeth == 0) {
+        RkCoef k{kRk1, grav, 0.0, 1.0, dt};
+        if ((rc = stage(h, h->U, nullptr, h->Ua, k, nullptr, true, false))) return rc;
+        std::swap(h->U, h->Ua);
+        h->prev_slot = &h->Ua;
+    } else if (rkmeth == 1) {
+        RkCoef k1{kRkStage, grav, 1.0, 0.0, 1.0 * dt};
+        RkCoef k2{kRkStage, grav, 0.5, 0.5, 0.5 * dt};
+        if ((rc = stage(h, h->U, h->U, h->Ua, k1, nullptr, false, false))) return rc;
+        if ((rc = stage(h, h->Ua, h->U, h->Ub, k2, nullptr, true))) return rc;
+        std::swap(h->U, h->Ub);
+        h->prev_slot = &h->Ub;
+    } else if (rkmeth == 2) {
+        RkCoef k1{kRkStage, grav, 1.0, 0.0, 1.0 * dt};
+        RkCoef k2{kRkStage, grav, 3.0 / 4.0, 1.0 / 4.0, (1.0 / 4.0) * dt};
+        RkCoef k3{kRkStage, grav, 1.0 / 3.0, 2.0 / 3.0, (2.0 / 3.0) * dt};
+        if ((rc = stage(h, h->U, h->U, h->Ua, k1, nullptr, false, false))) return rc;
+        if ((rc = stage(h, h->Ua, h->U, h->Ub, k2, nullptr, false))) return rc;
+        if ((rc = stage(h, h->Ub, h->U, h->Ua, k3, nullptr, true))) return rc;
+        std::swap(h->U, h->Ua);
+        h->prev_slot = &h->Ua;
+    } else {
+        return fail(WENOFD_E_BAD_ENUM, "RKMETH must be 0, 1 or 2");
+    }
+    h->prev_dt = dt;
+    return 0;
+}
+
+int run_rk(wenofd_solver* h, int rkmeth, double dt) {
+    if (h->sw_mode != WENOFD_SWITCH_OFF) return run_rk_switched(h, rkmeth, dt);
+    const int grav = h->opt.testcase == kRTI ? 1 : 0;
+    int rc;
+    h->dt_parts = 0;
+    h->prev_slot = nullptr;
     if (rkmeth == 0) {  // RunRK1, src/WENOFD.cpp:301-311: U += dt rhs
         RkCoef k{kRk1, grav, 0.0, 1.0, dt};
         if ((rc = stage(h, h->U, nullptr, h->Ua, k, nullptr, true))) return rc;
@@ -229,6 +308,7 @@ int upload_initial(wenofd_solver* h) {
         HIP_TRY(hipMemcpyAsync(b, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->dt_parts = 0;
+    h->prev_slot = nullptr;
     h->now = 0;
     h->steps = 0;
     return 0;
@@ -397,6 +477,11 @@ int wenofd_destroy(wenofd_solver* h) {
     (void)hipFree(h->dt_part);
     (void)hipFree(h->dt_part_nan);
     (void)hipFree(h->dt_dev);
+    (void)hipFree(h->sw_params);
+    (void)hipFree(h->fl_raw_x);
+    (void)hipFree(h->fl_raw_y);
+    (void)hipFree(h->fl_x);
+    (void)hipFree(h->fl_y);
     if (h->dt_host) (void)hipHostFree(h->dt_host);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -437,6 +522,10 @@ int wenofd_get_flux(wenofd_solver* h) {
         HIP_TRY(hipMemsetAsync(h->rhs, 0, h->state_doubles() * sizeof(double), h->stream));
     }
     RkCoef k{kRkNone, h->opt.testcase == kRTI ? 1 : 0, 0.0, 0.0, 0.0};
+    if (h->sw_mode != WENOFD_SWITCH_OFF) {
+        if ((rc = classify(h))) return rc;
+        return stage(h, h->U, nullptr, nullptr, k, h->rhs, false, false);
+    }
     return stage(h, h->U, nullptr, nullptr, k, h->rhs, false);
 }
 
@@ -475,6 +564,7 @@ int wenofd_set_state(wenofd_solver* h, const double* host, int layout) {
     HIP_TRY(hipMemcpyAsync(h->U, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->dt_parts = 0;
+    h->prev_slot = nullptr;
     return 0;
 }
 
@@ -483,7 +573,24 @@ int wenofd_step(wenofd_solver* h, double dt_fixed, double* dt_used) {
     int rc = set_dev(h);
     if (rc) return rc;
     double dt = dt_fixed;
-    if (!(dt_fixed > 0) && (rc = delta_t(h, &dt))) return rc;
+    if (!(dt_fixed > 0) && (rc = delta_t(h, &dt))) {
+        if (rc != WENOFD_E_NAN_DT || !h->prev_slot) return rc;
+        // The last (switched) step left a non-physical state: the plain solver
+        // redoes it from the state before it, then the step goes on.
+        std::swap(h->U, *h->prev_slot);
+        h->prev_slot = nullptr;
+        h->now -= h->prev_dt;
+        h->steps -= 1;
+        const int mode = h->sw_mode;
+        h->sw_mode = WENOFD_SWITCH_OFF;
+        rc = run_rk(h, h->opt.rkmeth, h->prev_dt);
+        h->sw_mode = mode;
+        if (rc) return rc;
+        h->now += h->prev_dt;
+        h->steps += 1;
+        h->fallbacks += 1;
+        if ((rc = delta_t(h, &dt))) return rc;
+    }
     if ((rc = run_rk(h, h->opt.rkmeth, dt))) return rc;
     h->now += dt;
     h->steps += 1;
@@ -679,6 +786,125 @@ int wenofd_synchronize(wenofd_solver* h) {
     int rc = set_dev(h);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int wenofd_switch_set(wenofd_solver* h, int mode, const float* params, float threshold) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    if (mode < WENOFD_SWITCH_OFF || mode > WENOFD_SWITCH_WENO) return fail(WENOFD_E_BAD_ENUM, "unknown switch mode");
+    if (mode == WENOFD_SWITCH_MLP && !params) return fail(WENOFD_E_ARG, "the MLP switch needs parameters");
+    int rc = set_dev(h);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (!h->fl_x) {
+        HIP_TRY(hipMalloc(&h->fl_raw_x, h->faces_x()));
+        HIP_TRY(hipMalloc(&h->fl_raw_y, h->faces_y()));
+        HIP_TRY(hipMalloc(&h->fl_x, h->faces_x()));
+        HIP_TRY(hipMalloc(&h->fl_y, h->faces_y()));
+    }
+    if (mode == WENOFD_SWITCH_MLP) {
+        if (!h->sw_params) HIP_TRY(hipMalloc(&h->sw_params, kNParams * sizeof(float)));
+        HIP_TRY(hipMemcpyAsync(h->sw_params, params, kNParams * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    } else if (mode != WENOFD_SWITCH_OFF) {
+        const unsigned char v = mode == WENOFD_SWITCH_WENO ? 1 : 0;
+        HIP_TRY(launch_fill_u8(h->fl_x, h->faces_x(), v, h->stream));
+        HIP_TRY(launch_fill_u8(h->fl_y, h->faces_y(), v, h->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->sw_mode = mode;
+    h->sw_thr = threshold;
+    h->prev_slot = nullptr;
+    return 0;
+}
+
+int wenofd_switch_classify(wenofd_solver* h, int dilate, unsigned char* flags_x, unsigned char* flags_y,
+                           float* logits_x, float* logits_y) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    if (h->sw_mode != WENOFD_SWITCH_MLP) return fail(WENOFD_E_ARG, "the switch is not in MLP mode");
+    int rc = set_dev(h);
+    if (rc) return rc;
+    float *lx = nullptr, *ly = nullptr;
+    if (logits_x) HIP_TRY(hipMalloc(&lx, h->faces_x() * 4 * sizeof(float)));
+    if (logits_y) {
+        hipError_t e = hipMalloc(&ly, h->faces_y() * 4 * sizeof(float));
+        if (e != hipSuccess) {
+            (void)hipFree(lx);
+            return fail(WENOFD_E_HIP, std::string("allocation: ") + hipGetErrorString(e));
+        }
+    }
+    rc = classify(h, lx, ly, dilate != 0);
+    const unsigned char* fx = dilate ? h->fl_x : h->fl_raw_x;
+    const unsigned char* fy = dilate ? h->fl_y : h->fl_raw_y;
+    hipError_t e = hipSuccess;
+    if (!rc && flags_x) e = hipMemcpyAsync(flags_x, fx, h->faces_x(), hipMemcpyDeviceToHost, h->stream);
+    if (!rc && e == hipSuccess && flags_y) e = hipMemcpyAsync(flags_y, fy, h->faces_y(), hipMemcpyDeviceToHost, h->stream);
+    if (!rc && e == hipSuccess && lx)
+        e = hipMemcpyAsync(logits_x, lx, h->faces_x() * 4 * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+    if (!rc && e == hipSuccess && ly)
+        e = hipMemcpyAsync(logits_y, ly, h->faces_y() * 4 * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(lx);
+    (void)hipFree(ly);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(WENOFD_E_HIP, std::string("switch_classify: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int wenofd_switch_flags(wenofd_solver* h, unsigned char* flags_x, unsigned char* flags_y) {
+    if (!h || !flags_x || !flags_y) return fail(WENOFD_E_ARG, "null argument");
+    if (!h->fl_x) return fail(WENOFD_E_ARG, "the switch was never enabled");
+    int rc = set_dev(h);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(flags_x, h->fl_x, h->faces_x(), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(flags_y, h->fl_y, h->faces_y(), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int wenofd_switch_stats(wenofd_solver* h, long long* fallbacks) {
+    if (!h || !fallbacks) return fail(WENOFD_E_ARG, "null argument");
+    *fallbacks = h->fallbacks;
+    return 0;
+}
+
+int wenofd_harvest(wenofd_solver* h, int dir, float* X, unsigned char* Y) {
+    if (!h || !X || !Y || dir < 0 || dir > 1) return fail(WENOFD_E_ARG, "bad argument");
+    int rc = set_dev(h);
+    if (rc) return rc;
+    const long nf = dir == 0 ? h->faces_x() : h->faces_y();
+    float* dX = nullptr;
+    unsigned char* dY = nullptr;
+    HIP_TRY(hipMalloc(&dX, nf * 4 * kFeat * sizeof(float)));
+    hipError_t e = hipMalloc(&dY, nf * 4);
+    if (e == hipSuccess) e = launch_ghost_fill(h->U, h->special, h->g, h->bc, h->stream);
+    if (e == hipSuccess) e = launch_harvest(dir, h->U, h->g, dX, dY, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(X, dX, nf * 4 * kFeat * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(Y, dY, nf * 4, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(dX);
+    (void)hipFree(dY);
+    if (e != hipSuccess) return fail(WENOFD_E_HIP, std::string("harvest: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int wenofd_mlp_harvest(wenofd_mlp* m, wenofd_solver* h, double keep_pos, double keep_neg, unsigned long long seed,
+                       long long* added) {
+    if (!m || !h) return fail(WENOFD_E_ARG, "null handle");
+    if (m->device != h->device) return fail(WENOFD_E_ARG, "trainer and solver live on different devices");
+    int rc = set_dev(h);
+    if (rc) return rc;
+    HIP_TRY(launch_ghost_fill(h->U, h->special, h->g, h->bc, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    long long total = 0;
+    for (int dir = 0; dir < 2; ++dir) {
+        const long nf = dir == 0 ? h->faces_x() : h->faces_y();
+        if ((rc = mlp_reserve_dense(m, nf))) return rc;
+        HIP_TRY(launch_harvest(dir, h->U, h->g, m->dense_X, m->dense_Y, m->stream));
+        long long got = 0;
+        if ((rc = mlp_append_dense(m, nf * 4, seed * 2 + dir, keep_pos, keep_neg, &got))) return rc;
+        total += got;
+    }
+    if (added) *added = total;
     return 0;
 }
 

@@ -18,6 +18,7 @@
 
 #include "wenofd_kernels.hpp"
 #include "wenofd_math.hpp"
+#include "wenofd_switch.hpp"
 
 namespace wenofd {
 
@@ -103,12 +104,38 @@ constexpr int kYC = 64;   // columns per tile
 constexpr int kYR = 4;    // face rows per step
 constexpr int kRing = 9;  // ring rows: a step reads 9 consecutive point rows
 
-template <int S, int YF>
+// Block-wide compaction of the flagged faces of one pass: slot `idx` of the
+// returned count receives the thread index of each flagged thread, in thread
+// order. Uses sCnt[blockDim/64] and sIdx[blockDim].
+template <int NT>
+__device__ __forceinline__ int compact_flagged(bool fl, int* sIdx, int* sCnt) {
+    const int t = threadIdx.x, lane = t % 64, w = t / 64;
+    const unsigned long long m = __ballot(fl);
+    const int pos = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) sCnt[w] = __popcll(m);
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int q = 0; q < NT / 64; ++q) {
+        base += q < w ? sCnt[q] : 0;
+        total += sCnt[q];
+    }
+    if (fl) sIdx[base + pos] = t;
+    __syncthreads();
+    return total;
+}
+
+// HYB: faces whose flag (flags[(f-4) nx + (i-4)]) is zero take the
+// component-wise upwind-5 flux; the flagged ones are gathered and computed
+// with the characteristic WENO flux by as few waves as they fill.
+template <int S, int YF, bool HYB = false>
 __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double* __restrict__ Fy, const Geom& g,
-                                            int bx, int by) {
+                                            int bx, int by, const unsigned char* __restrict__ flags = nullptr) {
     static_assert(YF % kYR == 0, "YF must be a multiple of the step height");
     __shared__ double sP[4][kRing][kYC];
     __shared__ double sM[4][kRing][kYC];
+    __shared__ int sIdx[HYB ? kYC * kYR : 1];
+    __shared__ int sCnt[kYR];
     const int tx = threadIdx.x % kYC;
     const int ty = threadIdx.x / kYC;
     const int i = 4 + bx * kYC + tx;
@@ -136,10 +163,10 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
     for (int f = F0; f < F1; f += kYR) {
         load_row(f + 2 + ty);  // rows f+2 .. f+5 complete the window [f-3, f+6)
         __syncthreads();
-        const int fr = f + ty;
-        if (col_ok && fr < F1) {
+        auto full_face = [&](int cx, int fr) {
+            const int ic = 4 + bx * kYC + cx;
             double Ul[4], Ur[4], Fh[4];
-            const size_t ol = (size_t)(fr - 1) * g.pitch + i;
+            const size_t ol = (size_t)(fr - 1) * g.pitch + ic;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 Ul[r] = U[r * g.plane + ol];
@@ -147,11 +174,34 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
             }
             const int s0 = fr - 3 - base;  // ring position of point row fr-3
             face_flux_dev<1, S>(
-                Ul, Ur, [&](int s, int r) { return sP[r][(s0 + s) % kRing][tx]; },
-                [&](int s, int r) { return sM[r][(s0 + s) % kRing][tx]; }, g.gamma, g.zp_lambda, Fh);
-            const size_t o = (size_t)(fr - 4) * g.fy_pitch + (i - 4);
+                Ul, Ur, [&](int s, int r) { return sP[r][(s0 + s) % kRing][cx]; },
+                [&](int s, int r) { return sM[r][(s0 + s) % kRing][cx]; }, g.gamma, g.zp_lambda, Fh);
+            const size_t o = (size_t)(fr - 4) * g.fy_pitch + (ic - 4);
 #pragma unroll
             for (int r = 0; r < 4; ++r) Fy[r * g.fy_plane + o] = Fh[r];
+        };
+        const int fr = f + ty;
+        if constexpr (HYB) {
+            bool fl = false;
+            if (col_ok && fr < F1) {
+                fl = flags[(size_t)(fr - 4) * g.nx + (i - 4)] != 0;
+                if (!fl) {
+                    const int s0 = fr - 3 - base;
+                    double Fh[4];
+                    face_flux_linear<double>([&](int s, int r) { return sP[r][(s0 + s) % kRing][tx]; },
+                                             [&](int s, int r) { return sM[r][(s0 + s) % kRing][tx]; }, Fh);
+                    const size_t o = (size_t)(fr - 4) * g.fy_pitch + (i - 4);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) Fy[r * g.fy_plane + o] = Fh[r];
+                }
+            }
+            const int nfl = compact_flagged<kYC * kYR>(fl, sIdx, sCnt);
+            for (int q = threadIdx.x; q < nfl; q += kYC * kYR) {
+                const int tt = sIdx[q];
+                full_face(tt % kYC, f + tt / kYC);
+            }
+        } else {
+            if (col_ok && fr < F1) full_face(tx, fr);
         }
         __syncthreads();
     }
@@ -162,9 +212,10 @@ __host__ __device__ inline int y_tiles_x(const Geom& g) { return (g.nx + kYC - 1
 template <int YF>
 __host__ __device__ inline int y_tiles_y(const Geom& g) { return (g.ny + 1 + YF - 1) / YF; }
 
-template <int S, int YF>
-__global__ __launch_bounds__(kYC* kYR) void k_flux_y(const double* __restrict__ U, double* __restrict__ Fy, Geom g) {
-    flux_y_tile<S, YF>(U, Fy, g, blockIdx.x, blockIdx.y);
+template <int S, int YF, bool HYB>
+__global__ __launch_bounds__(kYC* kYR) void k_flux_y(const double* __restrict__ U, double* __restrict__ Fy, Geom g,
+                                                     const unsigned char* __restrict__ flags) {
+    flux_y_tile<S, YF, HYB>(U, Fy, g, blockIdx.x, blockIdx.y, flags);
 }
 
 // ---------------------------------------------------------------------------
@@ -183,11 +234,12 @@ __global__ __launch_bounds__(kYC* kYR) void k_flux_y(const double* __restrict__ 
 // tile also reduces the local CFL step of its updated cells (the next
 // calculateDeltaT) into *part / *part_nan.
 // ---------------------------------------------------------------------------
-template <int S, int XT>
+template <int S, int XT, bool HYB = false>
 __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, const double* __restrict__ Fy,
                                                const double* Un, double* Uout, double* __restrict__ rhs_out,
                                                const Geom& g, const RkCoef& k, double cfl, double* part,
-                                               int* part_nan, int bx, int row) {
+                                               int* part_nan, int bx, int row,
+                                               const unsigned char* __restrict__ flags = nullptr) {
     constexpr int XC = XT - 1;  // cells per tile
     constexpr int XP = XT + 5;  // points per tile
     constexpr int XS = XT + 8;  // LDS row stride (slots)
@@ -219,18 +271,37 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
     }
     __syncthreads();
     const int f = f0 + t;
-    if (f <= g.nx + 4) {
+    auto full_face = [&](int tt) {
         double Ul[4], Ur[4], Fh[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            Ul[r] = sU[r][t + 2];
-            Ur[r] = sU[r][t + 3];
+            Ul[r] = sU[r][tt + 2];
+            Ur[r] = sU[r][tt + 3];
         }
         face_flux_dev<0, S>(
-            Ul, Ur, [&](int s, int r) { return sP[r][t + s]; }, [&](int s, int r) { return sM[r][t + s]; },
+            Ul, Ur, [&](int s, int r) { return sP[r][tt + s]; }, [&](int s, int r) { return sM[r][tt + s]; },
             g.gamma, g.zp_lambda, Fh);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) sF[r][t] = Fh[r];
+        for (int r = 0; r < 4; ++r) sF[r][tt] = Fh[r];
+    };
+    if constexpr (HYB) {
+        __shared__ int sIdx[XT];
+        __shared__ int sCnt[W];
+        bool fl = false;
+        if (f <= g.nx + 4) {
+            fl = flags[(size_t)row * (g.nx + 1) + (f - 4)] != 0;
+            if (!fl) {
+                double Fh[4];
+                face_flux_linear<double>([&](int s, int r) { return sP[r][t + s]; },
+                                         [&](int s, int r) { return sM[r][t + s]; }, Fh);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sF[r][t] = Fh[r];
+            }
+        }
+        const int nfl = compact_flagged<XT>(fl, sIdx, sCnt);
+        for (int q = t; q < nfl; q += XT) full_face(sIdx[q]);
+    } else {
+        if (f <= g.nx + 4) full_face(t);
     }
     const int i = f0 + t;  // this thread's cell
     const bool cell_ok = t < XC && i < g.nx + 4;
@@ -303,14 +374,14 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
 template <int XT>
 __host__ __device__ inline int x_tiles_x(const Geom& g) { return (g.nx + (XT - 1) - 1) / (XT - 1); }
 
-template <int S, int XT>
+template <int S, int XT, bool HYB>
 __global__ __launch_bounds__(XT) void k_flux_x_rk(const double* __restrict__ Uin, const double* __restrict__ Fy,
                                                   const double* Un, double* Uout, double* __restrict__ rhs_out,
                                                   Geom g, RkCoef k, double cfl, double* __restrict__ dt_part,
-                                                  int* __restrict__ dt_nan) {
+                                                  int* __restrict__ dt_nan, const unsigned char* __restrict__ flags) {
     const long b = (long)blockIdx.y * gridDim.x + blockIdx.x;
-    flux_x_rk_tile<S, XT>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part ? dt_part + b : nullptr,
-                          dt_part ? dt_nan + b : nullptr, blockIdx.x, blockIdx.y);
+    flux_x_rk_tile<S, XT, HYB>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part ? dt_part + b : nullptr,
+                               dt_part ? dt_nan + b : nullptr, blockIdx.x, blockIdx.y, flags);
 }
 
 // ---------------------------------------------------------------------------
@@ -620,23 +691,31 @@ hipError_t launch_ghost_fill(double* U, const double* special, const Geom& g, co
     return hipGetLastError();
 }
 
-hipError_t launch_flux_y(int scheme, const Tiles& t, const double* U, double* Fy, const Geom& g, hipStream_t s) {
+hipError_t launch_flux_y(int scheme, const Tiles& t, const double* U, double* Fy, const Geom& g, hipStream_t s,
+                         const unsigned char* flags) {
     return with_scheme(scheme, [&](auto S) {
         with_y_tile(t.y, [&](auto YF) {
             dim3 grid(y_tiles_x<YF()>(g), y_tiles_y<YF()>(g));
-            hipLaunchKernelGGL((k_flux_y<S(), YF()>), grid, dim3(kYC * kYR), 0, s, U, Fy, g);
+            if (flags)
+                hipLaunchKernelGGL((k_flux_y<S(), YF(), true>), grid, dim3(kYC * kYR), 0, s, U, Fy, g, flags);
+            else
+                hipLaunchKernelGGL((k_flux_y<S(), YF(), false>), grid, dim3(kYC * kYR), 0, s, U, Fy, g, flags);
         });
     });
 }
 
 hipError_t launch_flux_x_rk(int scheme, const Tiles& t, const double* Uin, const double* Fy, const double* Un,
                             double* Uout, double* rhs_out, const Geom& g, const RkCoef& k, double cfl,
-                            double* dt_part, int* dt_nan, hipStream_t s) {
+                            double* dt_part, int* dt_nan, hipStream_t s, const unsigned char* flags) {
     return with_scheme(scheme, [&](auto S) {
         with_x_tile(t.x, [&](auto XT) {
             dim3 grid(x_tiles_x<XT()>(g), g.ny);
-            hipLaunchKernelGGL((k_flux_x_rk<S(), XT()>), grid, dim3(XT()), 0, s, Uin, Fy, Un, Uout, rhs_out, g, k,
-                               cfl, dt_part, dt_nan);
+            if (flags)
+                hipLaunchKernelGGL((k_flux_x_rk<S(), XT(), true>), grid, dim3(XT()), 0, s, Uin, Fy, Un, Uout,
+                                   rhs_out, g, k, cfl, dt_part, dt_nan, flags);
+            else
+                hipLaunchKernelGGL((k_flux_x_rk<S(), XT(), false>), grid, dim3(XT()), 0, s, Uin, Fy, Un, Uout,
+                                   rhs_out, g, k, cfl, dt_part, dt_nan, flags);
         });
     });
 }

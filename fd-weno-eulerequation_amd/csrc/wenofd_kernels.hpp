@@ -42,12 +42,15 @@ constexpr int kProbePhysStride = 82;
 
 // ---- one solve --------------------------------------------------------------
 hipError_t launch_ghost_fill(double* U, const double* special, const Geom& g, const Bc& bc, hipStream_t s);
-hipError_t launch_flux_y(int scheme, const Tiles& t, const double* U, double* Fy, const Geom& g, hipStream_t s);
+// flags (optional): per-face switch flags of the direction (x faces
+// [ny][nx+1], y faces [ny+1][nx]); zero selects the upwind-5 flux.
+hipError_t launch_flux_y(int scheme, const Tiles& t, const double* U, double* Fy, const Geom& g, hipStream_t s,
+                         const unsigned char* flags = nullptr);
 // x faces + rhs + RK stage. dt_part (optional) receives one partial CFL
 // minimum per block: flux_x_rk_blocks(g, t) entries.
 hipError_t launch_flux_x_rk(int scheme, const Tiles& t, const double* Uin, const double* Fy, const double* Un,
                             double* Uout, double* rhs_out, const Geom& g, const RkCoef& k, double cfl,
-                            double* dt_part, int* dt_nan, hipStream_t s);
+                            double* dt_part, int* dt_nan, hipStream_t s, const unsigned char* flags = nullptr);
 long flux_x_rk_blocks(const Geom& g, const Tiles& t);
 long flux_y_blocks(const Geom& g, const Tiles& t);
 long ghost_blocks(const Geom& g);
@@ -113,5 +116,27 @@ hipError_t launch_batch_advance(const BatchView& b, bool count_only, hipStream_t
 // The tile table is the one of the fused x kernel for the same tile shape.
 hipError_t launch_batch_copy(const Tiles& t, const BatchView& b, int from, int to, const TileRef* tiles, int nblocks,
                              hipStream_t s);
+
+// ---- learned switch (wenofd_switch.hip) ----------------------------------------
+// Flags (and optionally logits [faces][4]) of every face of direction dir from
+// the ghost-filled state U; params = MLP parameters (float, device).
+hipError_t launch_classify(int dir, const double* U, const Geom& g, const float* params, float thr,
+                           unsigned char* flags, float* logits, hipStream_t s);
+// Features [faces][4][10] and labels [faces][4] of every face of direction dir.
+hipError_t launch_harvest(int dir, const double* U, const Geom& g, float* X, unsigned char* Y, hipStream_t s);
+hipError_t launch_dilate(const unsigned char* in, unsigned char* out, int rows, int cols, hipStream_t s);
+hipError_t launch_fill_u8(unsigned char* p, long n, unsigned char v, hipStream_t s);
+hipError_t launch_mlp_infer(const float* P, const float* X, long n, float* z, hipStream_t s);
+hipError_t launch_keep_count(const unsigned char* Y, long n, unsigned long long seed, double p_pos, double p_neg,
+                             int* counts, hipStream_t s);
+hipError_t launch_keep_write(const float* X, const unsigned char* Y, long n, unsigned long long seed, double p_pos,
+                             double p_neg, const long long* offsets, float* Xo, unsigned char* Yo, hipStream_t s);
+int mlp_grad_blocks(int n);
+// Mean loss and gradient over the n samples idx[0..n) of (X, Y);
+// partial: mlp_grad_blocks(n) x 1441 floats, lpart: mlp_grad_blocks(n) floats.
+hipError_t launch_mlp_grad(const float* P, const float* X, const unsigned char* Y, const int* idx, int n, float pw,
+                           float* partial, float* lpart, float* grad, float* loss, hipStream_t s);
+hipError_t launch_adam(float* P, const float* grad, float* m, float* v, float lr, float b1, float b2, float eps,
+                       float c1, float c2, hipStream_t s);
 
 }  // namespace wenofd

@@ -15,8 +15,10 @@ LIB_PATH = os.environ.get("WENOFD_LIBRARY") or os.path.join(HERE, "lib", "libwen
 
 OK, E_CONFIG, E_BAD_ENUM, E_NAN_DT, E_HIP, E_ARG, E_IO = range(7)
 LAYOUT_REF, LAYOUT_SOA, LAYOUT_REF_GHOSTED = 0, 1, 2
-K_GHOST, K_FLUX_X, K_FLUX_Y, K_RK, K_DT, K_COUNT = range(6)
-KERNEL_NAMES = ("ghost_fill", "flux_x_rk", "flux_y", "unused", "delta_t")
+K_GHOST, K_FLUX_X, K_FLUX_Y, K_SWITCH, K_DT, K_COUNT = range(6)
+KERNEL_NAMES = ("ghost_fill", "flux_x_rk", "flux_y", "switch", "delta_t")
+SWITCH_OFF, SWITCH_MLP, SWITCH_LINEAR, SWITCH_WENO = range(4)
+N_PARAMS = 1441
 
 # Exported symbols declared in include/wenofd_amd.h.
 EXPORTS = (
@@ -31,6 +33,10 @@ EXPORTS = (
     "wenofd_batch_advance", "wenofd_batch_set_graph_steps", "wenofd_batch_set_tiles", "wenofd_batch_get_tiles",
     "wenofd_batch_autotune", "wenofd_batch_get_state", "wenofd_batch_get_info", "wenofd_batch_accuracy",
     "wenofd_batch_synchronize",
+    "wenofd_switch_set", "wenofd_switch_classify", "wenofd_switch_flags", "wenofd_switch_stats", "wenofd_harvest",
+    "wenofd_mlp_create", "wenofd_mlp_destroy", "wenofd_mlp_set_params", "wenofd_mlp_get_params",
+    "wenofd_mlp_dataset_add", "wenofd_mlp_harvest", "wenofd_mlp_dataset_size", "wenofd_mlp_dataset_get",
+    "wenofd_mlp_dataset_clear", "wenofd_mlp_grad", "wenofd_mlp_train", "wenofd_mlp_infer", "wenofd_mlp_evaluate",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -100,6 +106,27 @@ def lib():
         L.wenofd_batch_get_state.argtypes = [vp, i, _dp, i]
         L.wenofd_batch_get_info.argtypes = [vp, i, ctypes.POINTER(Info)]
         L.wenofd_batch_accuracy.argtypes = [vp, i, _dp]
+        fp, u8, ll, ull = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_ubyte), ctypes.c_longlong, \
+            ctypes.c_ulonglong
+        llp = ctypes.POINTER(ll)
+        L.wenofd_switch_set.argtypes = [vp, i, fp, ctypes.c_float]
+        L.wenofd_switch_classify.argtypes = [vp, i, u8, u8, fp, fp]
+        L.wenofd_switch_flags.argtypes = [vp, u8, u8]
+        L.wenofd_switch_stats.argtypes = [vp, llp]
+        L.wenofd_harvest.argtypes = [vp, i, fp, u8]
+        L.wenofd_mlp_create.argtypes = [i, ull, ctypes.POINTER(vp)]
+        L.wenofd_mlp_destroy.argtypes = [vp]
+        L.wenofd_mlp_set_params.argtypes = [vp, fp]
+        L.wenofd_mlp_get_params.argtypes = [vp, fp]
+        L.wenofd_mlp_dataset_add.argtypes = [vp, fp, u8, ll]
+        L.wenofd_mlp_harvest.argtypes = [vp, vp, d, d, ull, llp]
+        L.wenofd_mlp_dataset_size.argtypes = [vp, llp, llp]
+        L.wenofd_mlp_dataset_get.argtypes = [vp, ll, ll, fp, u8]
+        L.wenofd_mlp_dataset_clear.argtypes = [vp]
+        L.wenofd_mlp_grad.argtypes = [vp, ctypes.POINTER(i), i, ctypes.c_float, fp, fp]
+        L.wenofd_mlp_train.argtypes = [vp, i, i, ctypes.c_float, ctypes.c_float, ull, fp]
+        L.wenofd_mlp_infer.argtypes = [vp, fp, ll, fp]
+        L.wenofd_mlp_evaluate.argtypes = [vp, ll, ll, ctypes.c_float, llp]
         _LIB = L
     return _LIB
 
@@ -111,6 +138,14 @@ def _check(rc):
 
 def _ptr(a):
     return a.ctypes.data_as(_dp)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _u8ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
 
 
 def config_text(**kw):
@@ -279,6 +314,134 @@ class Solver:
 
     def synchronize(self):
         _check(lib().wenofd_synchronize(self.h))
+
+    # ---- learned switch
+    def faces(self, direction):
+        return self.ny * (self.nx + 1) if direction == 0 else (self.ny + 1) * self.nx
+
+    def switch_set(self, mode, params=None, threshold=0.0):
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float32).reshape(N_PARAMS)
+        _check(lib().wenofd_switch_set(self.h, int(mode), None if p is None else _fptr(p), float(threshold)))
+
+    def switch_classify(self, dilate=True, logits=False):
+        """(flags_x [NY][NX+1], flags_y [NY+1][NX], logits_x, logits_y) of the current state."""
+        fx = np.empty((self.ny, self.nx + 1), np.uint8)
+        fy = np.empty((self.ny + 1, self.nx), np.uint8)
+        lx = np.empty((self.ny, self.nx + 1, 4), np.float32) if logits else None
+        ly = np.empty((self.ny + 1, self.nx, 4), np.float32) if logits else None
+        _check(lib().wenofd_switch_classify(self.h, 1 if dilate else 0, _u8ptr(fx), _u8ptr(fy),
+                                            None if lx is None else _fptr(lx), None if ly is None else _fptr(ly)))
+        return fx, fy, lx, ly
+
+    def switch_flags(self):
+        fx = np.empty((self.ny, self.nx + 1), np.uint8)
+        fy = np.empty((self.ny + 1, self.nx), np.uint8)
+        _check(lib().wenofd_switch_flags(self.h, _u8ptr(fx), _u8ptr(fy)))
+        return fx, fy
+
+    def switch_fallbacks(self):
+        n = ctypes.c_longlong()
+        _check(lib().wenofd_switch_stats(self.h, ctypes.byref(n)))
+        return n.value
+
+    def harvest(self, direction):
+        nf = self.faces(direction)
+        X = np.empty((nf, 4, 10), np.float32)
+        Y = np.empty((nf, 4), np.uint8)
+        _check(lib().wenofd_harvest(self.h, int(direction), _fptr(X), _u8ptr(Y)))
+        return X, Y
+
+
+class Mlp:
+    """Switch MLP trainer (wenofd_mlp handle): parameters, Adam state, samples on the GPU."""
+
+    def __init__(self, seed=0, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().wenofd_mlp_create(device, ctypes.c_ulonglong(seed), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().wenofd_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def params(self):
+        P = np.empty(N_PARAMS, np.float32)
+        _check(lib().wenofd_mlp_get_params(self.h, _fptr(P)))
+        return P
+
+    @params.setter
+    def params(self, P):
+        P = np.ascontiguousarray(P, dtype=np.float32).reshape(N_PARAMS)
+        _check(lib().wenofd_mlp_set_params(self.h, _fptr(P)))
+
+    def add(self, X, Y):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 10)
+        Y = np.ascontiguousarray(Y, dtype=np.uint8).reshape(-1)
+        _check(lib().wenofd_mlp_dataset_add(self.h, _fptr(X), _u8ptr(Y), X.shape[0]))
+
+    def harvest(self, solver, keep_pos=1.0, keep_neg=1.0, seed=0):
+        n = ctypes.c_longlong()
+        _check(lib().wenofd_mlp_harvest(self.h, solver.h, float(keep_pos), float(keep_neg),
+                                        ctypes.c_ulonglong(seed), ctypes.byref(n)))
+        return n.value
+
+    def size(self):
+        n, p = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().wenofd_mlp_dataset_size(self.h, ctypes.byref(n), ctypes.byref(p)))
+        return n.value, p.value
+
+    def samples(self, start=0, n=None):
+        if n is None:
+            n = self.size()[0] - start
+        X = np.empty((n, 10), np.float32)
+        Y = np.empty(n, np.uint8)
+        _check(lib().wenofd_mlp_dataset_get(self.h, int(start), int(n), _fptr(X), _u8ptr(Y)))
+        return X, Y
+
+    def clear(self):
+        _check(lib().wenofd_mlp_dataset_clear(self.h))
+
+    def grad(self, idx, pos_weight=1.0):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        G = np.empty(N_PARAMS, np.float32)
+        loss = np.empty(1, np.float32)
+        _check(lib().wenofd_mlp_grad(self.h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), idx.size,
+                                     float(pos_weight), _fptr(G), _fptr(loss)))
+        return float(loss[0]), G
+
+    def train(self, epochs, batch=16384, lr=3e-3, pos_weight=1.0, seed=0):
+        out = np.empty(epochs, np.float32)
+        _check(lib().wenofd_mlp_train(self.h, int(epochs), int(batch), float(lr), float(pos_weight),
+                                      ctypes.c_ulonglong(seed), _fptr(out)))
+        return out
+
+    def infer(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 10)
+        z = np.empty(X.shape[0], np.float32)
+        _check(lib().wenofd_mlp_infer(self.h, _fptr(X), X.shape[0], _fptr(z)))
+        return z
+
+    def evaluate(self, start=0, n=None, threshold=0.0):
+        """{TP, FP, FN, TN} of the deployed (bf16 MFMA) network."""
+        if n is None:
+            n = self.size()[0] - start
+        c = (ctypes.c_longlong * 4)()
+        _check(lib().wenofd_mlp_evaluate(self.h, int(start), int(n), float(threshold), c))
+        return dict(zip(("tp", "fp", "fn", "tn"), (int(v) for v in c)))
 
 
 class Batch:

@@ -49,9 +49,17 @@ enum {
     WENOFD_K_GHOST = 0,   /* ghost fill */
     WENOFD_K_FLUX_X = 1,  /* x-face fluxes fused with rhs and the RK stage update */
     WENOFD_K_FLUX_Y = 2,  /* y-face fluxes */
-    WENOFD_K_RK = 3,      /* reserved (no separate RK kernel) */
+    WENOFD_K_SWITCH = 3,  /* switch classification (MLP inference + flag dilation) */
     WENOFD_K_DT = 4,      /* CFL time-step reduction */
     WENOFD_K_COUNT = 5
+};
+
+/* Learned troubled-stencil switch modes (wenofd_switch_set). */
+enum {
+    WENOFD_SWITCH_OFF = 0,    /* the reference's characteristic WENO flux at every face */
+    WENOFD_SWITCH_MLP = 1,    /* flags from the switch MLP; unflagged faces take upwind-5 */
+    WENOFD_SWITCH_LINEAR = 2, /* every face takes the component-wise upwind-5 flux */
+    WENOFD_SWITCH_WENO = 3    /* every face flagged (the hybrid kernels, plain results) */
 };
 
 typedef struct wenofd_info {
@@ -145,6 +153,66 @@ int wenofd_synchronize(wenofd_solver* h);
  * kernel (16, 32 or 64). Results are bitwise independent of the shape. No
  * counterpart in the reference (a launch-shape knob of this implementation). */
 int wenofd_set_tiles(wenofd_solver* h, int x_faces, int y_faces);
+
+/*
+ * Learned troubled-stencil switch. No counterpart in the reference. At the
+ * start of every step the switch MLP (10 -> 32 -> 32 -> 1, GELU; bf16 MFMA
+ * inference, FP32 accumulate) reads the normalised 5-point stencils of each
+ * characteristic field of each face and flags the faces whose WENO-Z weights
+ * it predicts to leave the linear weights (logit > threshold for any field).
+ * The flags are dilated by one face and serve all stages of the step: flagged
+ * faces take the reference's characteristic WENO flux, the others the cheap
+ * component-wise linear upwind-5 flux. The plain solver is the fallback: a
+ * switched step that leads to a NaN time step is redone without the switch.
+ * params: 1441 floats, W1[32][10], b1[32], W2[32][32], b2[32], w3[32], b3.
+ */
+int wenofd_switch_set(wenofd_solver* h, int mode, const float* params, float threshold);
+/* Classifies the current state (ghost fill first): flags per face, dilated
+ * (dilate != 0) or raw, and the logits [faces][4]; any output may be null.
+ * x faces are [NY][NX+1] (row j, face f between points f-1 and f), y faces
+ * [NY+1][NX]. */
+int wenofd_switch_classify(wenofd_solver* h, int dilate, unsigned char* flags_x, unsigned char* flags_y,
+                           float* logits_x, float* logits_y);
+/* The (dilated) flags the last switched step used. */
+int wenofd_switch_flags(wenofd_solver* h, unsigned char* flags_x, unsigned char* flags_y);
+/* Number of switched steps the plain solver had to redo. */
+int wenofd_switch_stats(wenofd_solver* h, long long* fallbacks);
+/* Switch samples of every face of one direction of the current state:
+ * features [faces][4][10] (per characteristic field: the + stencil and the
+ * mirrored - stencil, each centred and scaled) and labels [faces][4]
+ * (max |omega - d| of WENO-Z over both sides > 0.05). */
+int wenofd_harvest(wenofd_solver* h, int dir, float* X, unsigned char* Y);
+
+/* Switch MLP trainer: parameters, Adam moments and a device-resident sample
+ * set; FP32 forward/backward/Adam kernels of this library. */
+typedef struct wenofd_mlp wenofd_mlp;
+/* Parameters initialised ~ U(-1/sqrt(fan_in), 1/sqrt(fan_in)) from seed. */
+int wenofd_mlp_create(int device, unsigned long long seed, wenofd_mlp** out);
+int wenofd_mlp_destroy(wenofd_mlp* m);
+/* Sets the parameters (and clears the Adam moments) / reads them. */
+int wenofd_mlp_set_params(wenofd_mlp* m, const float* params);
+int wenofd_mlp_get_params(wenofd_mlp* m, float* params);
+/* Appends samples from host arrays X [n][10], Y [n]. */
+int wenofd_mlp_dataset_add(wenofd_mlp* m, const float* X, const unsigned char* Y, long long n);
+/* Appends the samples of both directions of a solver's current state, each
+ * kept with probability keep_pos (label 1) or keep_neg (label 0) from a
+ * seeded hash (deterministic). */
+int wenofd_mlp_harvest(wenofd_mlp* m, wenofd_solver* h, double keep_pos, double keep_neg, unsigned long long seed,
+                       long long* added);
+int wenofd_mlp_dataset_size(wenofd_mlp* m, long long* n, long long* positives);
+int wenofd_mlp_dataset_get(wenofd_mlp* m, long long start, long long n, float* X, unsigned char* Y);
+int wenofd_mlp_dataset_clear(wenofd_mlp* m);
+/* Mean loss (weighted BCE with logits: pw y softplus(-z) + (1-y) softplus(z))
+ * and its gradient at the current parameters over samples idx[0..n). */
+int wenofd_mlp_grad(wenofd_mlp* m, const int* idx, int n, float pos_weight, float* grad, float* loss);
+/* Adam (lr, 0.9, 0.999, 1e-8) over shuffled mini-batches; epoch_losses[e] =
+ * mean batch loss of epoch e (may be null). */
+int wenofd_mlp_train(wenofd_mlp* m, int epochs, int batch, float lr, float pos_weight, unsigned long long seed,
+                     float* epoch_losses);
+/* Logits of the deployed network (bf16 MFMA, FP32 accumulate) for X [n][10]. */
+int wenofd_mlp_infer(wenofd_mlp* m, const float* X, long long n, float* logits);
+/* {TP, FP, FN, TN} of the deployed network over samples [start, start+n). */
+int wenofd_mlp_evaluate(wenofd_mlp* m, long long start, long long n, float thr, long long* counts4);
 
 /*
  * Batched solves. No counterpart in the reference, which runs one case per

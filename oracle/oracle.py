@@ -58,6 +58,21 @@ def lib():
         L.oracle_now.argtypes = [vp]
         L.oracle_accuracy.argtypes = [vp, _dp]
         L.oracle_count_ops.argtypes = [i, i, i, i, i, ctypes.POINTER(ctypes.c_ulonglong)]
+        fp = ctypes.POINTER(ctypes.c_float)
+        u8 = ctypes.POINTER(ctypes.c_ubyte)
+        L.oracle_linear5.restype = d
+        L.oracle_linear5.argtypes = [d, d, d, d, d]
+        L.oracle_stencil_features.argtypes = [_dp, fp]
+        L.oracle_weno_z_departure.restype = d
+        L.oracle_weno_z_departure.argtypes = [_dp]
+        L.oracle_samples.argtypes = [vp, i, fp, u8]
+        L.oracle_set_flags.argtypes = [vp, u8, u8]
+        L.oracle_mlp_loss_grad.restype = d
+        L.oracle_mlp_loss_grad.argtypes = [fp, fp, u8, i, d, _dp]
+        L.oracle_mlp_loss_grad_d.restype = d
+        L.oracle_mlp_loss_grad_d.argtypes = [_dp, fp, u8, i, d, _dp]
+        L.oracle_mlp_forward.argtypes = [fp, fp, i, _dp]
+        L.oracle_mlp_infer_bf16.argtypes = [fp, fp, i, fp]
         _LIB = L
     return _LIB
 
@@ -101,6 +116,63 @@ def face_flux(direction, scheme, U6, gamma=1.4, zp_lambda=0.0):
     out = np.empty(4)
     lib().oracle_face_flux(int(direction), int(scheme), gamma, zp_lambda, _ptr(U6), _ptr(out))
     return out
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _u8ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
+
+
+N_PARAMS = 1441  # W1[32][10], b1[32], W2[32][32], b2[32], w3[32], b3
+
+
+def linear5(a, b, c, d, e):
+    return lib().oracle_linear5(float(a), float(b), float(c), float(d), float(e))
+
+
+def stencil_features(w5):
+    w = np.ascontiguousarray(w5, dtype=np.float64)
+    out = np.empty(5, dtype=np.float32)
+    lib().oracle_stencil_features(_ptr(w), _fptr(out))
+    return out
+
+
+def weno_z_departure(w5):
+    w = np.ascontiguousarray(w5, dtype=np.float64)
+    return lib().oracle_weno_z_departure(_ptr(w))
+
+
+def mlp_loss_grad(params, X, Y, pos_weight=1.0):
+    """Mean weighted BCE loss and its gradient (FP64) at float or double params."""
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 10)
+    Y = np.ascontiguousarray(Y, dtype=np.uint8).reshape(-1)
+    G = np.empty(N_PARAMS)
+    if np.asarray(params).dtype == np.float64:
+        P = np.ascontiguousarray(params, dtype=np.float64)
+        loss = lib().oracle_mlp_loss_grad_d(_ptr(P), _fptr(X), _u8ptr(Y), X.shape[0], float(pos_weight), _ptr(G))
+    else:
+        P = np.ascontiguousarray(params, dtype=np.float32)
+        loss = lib().oracle_mlp_loss_grad(_fptr(P), _fptr(X), _u8ptr(Y), X.shape[0], float(pos_weight), _ptr(G))
+    return loss, G
+
+
+def mlp_forward(params, X):
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 10)
+    z = np.empty(X.shape[0])
+    lib().oracle_mlp_forward(_fptr(P), _fptr(X), X.shape[0], _ptr(z))
+    return z
+
+
+def mlp_infer_bf16(params, X):
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 10)
+    z = np.empty(X.shape[0], dtype=np.float32)
+    lib().oracle_mlp_infer_bf16(_fptr(P), _fptr(X), X.shape[0], _fptr(z))
+    return z
 
 
 def seeded_quadrant(seed):
@@ -182,6 +254,26 @@ class Solver:
 
     def now(self):
         return lib().oracle_now(self.h)
+
+    def samples(self, direction):
+        """Ghost fill, then switch features [faces, 4, 10] and labels [faces, 4]
+        of every face of one direction (x: NY x (NX+1) faces, y: (NY+1) x NX)."""
+        nf = self.ny * (self.nx + 1) if direction == 0 else (self.ny + 1) * self.nx
+        X = np.empty((nf, 4, 10), dtype=np.float32)
+        Y = np.empty((nf, 4), dtype=np.uint8)
+        lib().oracle_samples(self.h, int(direction), _fptr(X), _u8ptr(Y))
+        return X, Y
+
+    def set_flags(self, flags_x, flags_y):
+        """Hybrid flux: faces with a zero flag take the upwind-5 flux. None restores."""
+        if flags_x is None:
+            self._flags = None
+            lib().oracle_set_flags(self.h, None, None)
+            return
+        fx = np.ascontiguousarray(flags_x, dtype=np.uint8).reshape(self.ny, self.nx + 1)
+        fy = np.ascontiguousarray(flags_y, dtype=np.uint8).reshape(self.ny + 1, self.nx)
+        self._flags = (fx, fy)  # kept alive while the oracle holds the pointers
+        lib().oracle_set_flags(self.h, _u8ptr(fx), _u8ptr(fy))
 
     def accuracy(self):
         out = np.empty(3)

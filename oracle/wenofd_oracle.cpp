@@ -357,6 +357,204 @@ static inline void face_flux(const R* Ul, const R* Ur, const R (*Fp5)[4], const 
 }
 
 // ---------------------------------------------------------------------------
+// Learned troubled-stencil switch (no reference counterpart; this oracle
+// defines it). Pinned by closed forms in tests/test_oracle_switch.py: the
+// upwind-5 formula equals WENO5 with omega == d and is exact on polynomials of
+// degree <= 4; features are invariant to v -> a v + b; labels are 0 on smooth
+// data and 1 across a jump; the MLP gradient equals central differences.
+// ---------------------------------------------------------------------------
+template <class R>
+static inline R linear5(R a, R b, R c, R d, R e) {
+    R t = R(2.0) * a;
+    t = t - R(13.0) * b;
+    t = t + R(47.0) * c;
+    t = t + R(27.0) * d;
+    t = t - R(3.0) * e;
+    return t * R(1.0 / 60.0);
+}
+
+// Mean-centred 5-point stencil scaled by 1 / (max |deviation| + 1e-10), as floats.
+static inline void stencil_features(const double w[5], float* x) {
+    double m = w[0];
+    for (int j = 1; j < 5; ++j) m = m + w[j];
+    m = m * 0.2;
+    double d[5], s = 0.0;
+    for (int j = 0; j < 5; ++j) {
+        d[j] = w[j] - m;
+        const double a = std::fabs(d[j]);
+        if (j == 0 || a > s) s = a;
+    }
+    const double inv = 1.0 / (s + 1e-10);
+    for (int j = 0; j < 5; ++j) x[j] = (float)(d[j] * inv);
+}
+
+// max_j |omega_j - d_j| for WENO-Z on (a..e), include/Utility.hpp:457-496.
+static inline double weno_z_departure(double a, double b, double c, double d, double e) {
+    double b0, b1, b2;
+    smoothness<double>(a, b, c, d, e, b0, b1, b2);
+    const double eps = 1e-20;
+    const double i0 = 1.0 / (b0 + eps), i1 = 1.0 / (b1 + eps), i2 = 1.0 / (b2 + eps);
+    const double tau = std::fabs(b0 - b2);
+    const double a0 = 0.1 * (1.0 + (((tau * i0) * tau) * i0));
+    const double a1 = 0.6 * (1.0 + (((tau * i1) * tau) * i1));
+    const double a2 = 0.3 * (1.0 + (((tau * i2) * tau) * i2));
+    const double sum = (a0 + a1) + a2;
+    const double e0 = std::fabs(a0 / sum - 0.1), e1 = std::fabs(a1 / sum - 0.6), e2 = std::fabs(a2 / sum - 0.3);
+    double m = e0;
+    if (e1 > m) m = e1;
+    if (e2 > m) m = e2;
+    return m;
+}
+
+// Characteristic stencils of one face (the first half of face_flux).
+static inline void face_char(const double* Ul, const double* Ur, const double (*Fp5)[4], const double (*Fm5)[4],
+                             int dir, double gamma, double vp[5][4], double vm[5][4]) {
+    double ave[4], L[16];
+    for (int r = 0; r < 4; ++r) ave[r] = 0.5 * (Ul[r] + Ur[r]);
+    eigen_left(ave, dir, gamma, L);
+    for (int s = 0; s < 5; ++s)
+        for (int i = 0; i < 4; ++i) {
+            double a = L[i * 4 + 0] * Fp5[s][0];
+            a += L[i * 4 + 1] * Fp5[s][1];
+            a += L[i * 4 + 2] * Fp5[s][2];
+            a += L[i * 4 + 3] * Fp5[s][3];
+            vp[s][i] = a;
+            double b = L[i * 4 + 0] * Fm5[s][0];
+            b += L[i * 4 + 1] * Fm5[s][1];
+            b += L[i * 4 + 2] * Fm5[s][2];
+            b += L[i * 4 + 3] * Fm5[s][3];
+            vm[s][i] = b;
+        }
+}
+
+// Features x[k][0..9] (+ stencil, then the mirrored - stencil) and labels of a face.
+static inline void face_samples(const double vp[5][4], const double vm[5][4], float* x40, unsigned char* lab4) {
+    for (int k = 0; k < 4; ++k) {
+        const double wp[5] = {vp[0][k], vp[1][k], vp[2][k], vp[3][k], vp[4][k]};
+        const double wm[5] = {vm[4][k], vm[3][k], vm[2][k], vm[1][k], vm[0][k]};
+        stencil_features(wp, x40 + 10 * k);
+        stencil_features(wm, x40 + 10 * k + 5);
+        const double dp = weno_z_departure(wp[0], wp[1], wp[2], wp[3], wp[4]);
+        const double dm = weno_z_departure(wm[0], wm[1], wm[2], wm[3], wm[4]);
+        lab4[k] = (dp > 0.05 || dm > 0.05) ? 1 : 0;
+    }
+}
+
+// Component-wise linear upwind-5 face flux (the switch's cheap branch).
+template <class R>
+static inline void face_flux_linear(const R (*Fp5)[4], const R (*Fm5)[4], R* Fhat) {
+    for (int r = 0; r < 4; ++r) {
+        const R hp = linear5<R>(Fp5[0][r], Fp5[1][r], Fp5[2][r], Fp5[3][r], Fp5[4][r]);
+        const R hm = linear5<R>(Fm5[4][r], Fm5[3][r], Fm5[2][r], Fm5[1][r], Fm5[0][r]);
+        Fhat[r] = hp + hm;
+    }
+}
+
+// MLP 10 -> 32 -> 32 -> 1, tanh-form GELU, parameters as floats:
+// W1[32][10], b1[32], W2[32][32], b2[32], w3[32], b3.
+enum { P_W1 = 0, P_B1 = 320, P_W2 = 352, P_B2 = 1376, P_W3 = 1408, P_B3 = 1440, P_N = 1441 };
+static const double kGk0 = (double)0.7978845608028654f, kGk1 = (double)0.044715f;
+
+static inline double gelu(double x) { return 0.5 * x * (1.0 + std::tanh(kGk0 * (x + kGk1 * x * x * x))); }
+static inline double gelu_grad(double x) {
+    const double t = std::tanh(kGk0 * (x + kGk1 * x * x * x));
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * kGk0 * (1.0 + 3.0 * kGk1 * x * x);
+}
+
+struct MlpCache {
+    double a1[32], h1[32], a2[32], h2[32], z;
+};
+
+static inline double mlp_forward(const double* P, const float* x, MlpCache& c) {
+    for (int u = 0; u < 32; ++u) {
+        double a = P[P_B1 + u];
+        for (int k = 0; k < 10; ++k) a += P[P_W1 + u * 10 + k] * (double)x[k];
+        c.a1[u] = a;
+        c.h1[u] = gelu(a);
+    }
+    for (int u = 0; u < 32; ++u) {
+        double a = P[P_B2 + u];
+        for (int k = 0; k < 32; ++k) a += P[P_W2 + u * 32 + k] * c.h1[k];
+        c.a2[u] = a;
+        c.h2[u] = gelu(a);
+    }
+    double z = P[P_B3];
+    for (int k = 0; k < 32; ++k) z += P[P_W3 + k] * c.h2[k];
+    c.z = z;
+    return z;
+}
+
+// Weighted binary cross-entropy with logits: pw y softplus(-z) + (1 - y) softplus(z).
+static inline double softplus(double z) { return (z > 0 ? z : 0.0) + std::log1p(std::exp(-std::fabs(z))); }
+
+// Mean loss over n samples and its gradient (accumulated into G, size P_N).
+static double mlp_loss_grad(const double* P, const float* X, const unsigned char* Y, int n, double pw, double* G) {
+    double loss = 0;
+    for (int q = 0; q < P_N; ++q) G[q] = 0;
+    for (int s = 0; s < n; ++s) {
+        MlpCache c;
+        const double z = mlp_forward(P, X + 10 * (size_t)s, c);
+        const double y = Y[s] ? 1.0 : 0.0;
+        loss += pw * y * softplus(-z) + (1.0 - y) * softplus(z);
+        const double sg = 1.0 / (1.0 + std::exp(-z));
+        const double dz = (1.0 - y) * sg - pw * y * (1.0 - sg);
+        double da2[32], dh1[32] = {0};
+        G[P_B3] += dz;
+        for (int k = 0; k < 32; ++k) {
+            G[P_W3 + k] += dz * c.h2[k];
+            da2[k] = dz * P[P_W3 + k] * gelu_grad(c.a2[k]);
+        }
+        for (int u = 0; u < 32; ++u) {
+            G[P_B2 + u] += da2[u];
+            for (int k = 0; k < 32; ++k) {
+                G[P_W2 + u * 32 + k] += da2[u] * c.h1[k];
+                dh1[k] += P[P_W2 + u * 32 + k] * da2[u];
+            }
+        }
+        for (int u = 0; u < 32; ++u) {
+            const double da1 = dh1[u] * gelu_grad(c.a1[u]);
+            G[P_B1 + u] += da1;
+            for (int k = 0; k < 10; ++k) G[P_W1 + u * 10 + k] += da1 * (double)X[10 * (size_t)s + k];
+        }
+    }
+    for (int q = 0; q < P_N; ++q) G[q] /= n;
+    return loss / n;
+}
+
+// bfloat16 round-to-nearest-even of a float (finite inputs).
+static inline float bf16_round(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+
+// Inference as the GPU runs it: features, hidden activations and weights
+// rounded to bfloat16, products accumulated in float, biases in float.
+static inline float mlp_infer_bf16(const float* P, const float* x) {
+    float xb[10], h1[32], h2[32];
+    for (int k = 0; k < 10; ++k) xb[k] = bf16_round(x[k]);
+    for (int u = 0; u < 32; ++u) {
+        float a = 0;
+        for (int k = 0; k < 10; ++k) a += bf16_round(P[P_W1 + u * 10 + k]) * xb[k];
+        a += P[P_B1 + u];
+        h1[u] = bf16_round((float)gelu((double)a));
+    }
+    for (int u = 0; u < 32; ++u) {
+        float a = 0;
+        for (int k = 0; k < 32; ++k) a += bf16_round(P[P_W2 + u * 32 + k]) * h1[k];
+        a += P[P_B2 + u];
+        h2[u] = (float)gelu((double)a);
+    }
+    float z = 0;
+    for (int k = 0; k < 32; ++k) z += P[P_W3 + k] * h2[k];
+    return z + P[P_B3];
+}
+
+// ---------------------------------------------------------------------------
 // Test cases. utils/equations.hpp:400-662 plus the two cases that have no
 // reference counterpart (isentropic vortex, parameterised four-quadrant).
 // ---------------------------------------------------------------------------
@@ -510,6 +708,11 @@ struct Solver {
     double dx = 0, dy = 0, zp_lambda = 0, now = 0;
     std::vector<double> xc, yc;  // node coordinates incl. ghosts
     std::vector<R> Uh, Un, rhs;
+    // Switch flags (null: every face takes face_flux): x faces [NY][NX+1]
+    // (row j-4, face f-4), y faces [NY+1][NX] (face f-4, column i-4); nonzero
+    // = characteristic WENO, zero = component-wise upwind-5.
+    const unsigned char* flags_x = nullptr;
+    const unsigned char* flags_y = nullptr;
 
     size_t idx(int i, int j) const { return ((size_t)i * TY + j) * 4; }
 
@@ -617,6 +820,7 @@ struct Solver {
 
     void fill_ghosts() { exchange(); set_boundary(); }
 
+
     // assembleBoundaryTerm minus the halo (WENOFD.cpp:481-798); rhs += flux terms.
     void flux_terms() {
         const int sx = g, ex = g + NX, sy = g, ey = g + NY;
@@ -630,7 +834,11 @@ struct Solver {
                         fp5[s + 2][r] = Fp[(f + s - 1) * 4 + r];
                         fm5[s + 2][r] = Fm[(f + s) * 4 + r];
                     }
-                face_flux<R>(&Uh[idx(f - 1, j)], &Uh[idx(f, j)], fp5, fm5, 0, scheme, cs.gamma, zp_lambda, &Fh[f * 4]);
+                if (flags_x && !flags_x[(size_t)(j - sy) * (NX + 1) + (f - sx)])
+                    face_flux_linear(fp5, fm5, &Fh[f * 4]);
+                else
+                    face_flux<R>(&Uh[idx(f - 1, j)], &Uh[idx(f, j)], fp5, fm5, 0, scheme, cs.gamma, zp_lambda,
+                                 &Fh[f * 4]);
             }
             for (int i = sx; i < ex; ++i)
                 for (int r = 0; r < 4; ++r) rhs[idx(i, j) + r] += -(Fh[(i + 1) * 4 + r] - Fh[i * 4 + r]) / R(dx);
@@ -644,10 +852,44 @@ struct Solver {
                         fp5[s + 2][r] = Fp[(f + s - 1) * 4 + r];
                         fm5[s + 2][r] = Fm[(f + s) * 4 + r];
                     }
-                face_flux<R>(&Uh[idx(i, f - 1)], &Uh[idx(i, f)], fp5, fm5, 1, scheme, cs.gamma, zp_lambda, &Fh[f * 4]);
+                if (flags_y && !flags_y[(size_t)(f - sy) * NX + (i - sx)])
+                    face_flux_linear(fp5, fm5, &Fh[f * 4]);
+                else
+                    face_flux<R>(&Uh[idx(i, f - 1)], &Uh[idx(i, f)], fp5, fm5, 1, scheme, cs.gamma, zp_lambda,
+                                 &Fh[f * 4]);
             }
             for (int j = sy; j < ey; ++j)
                 for (int r = 0; r < 4; ++r) rhs[idx(i, j) + r] += -(Fh[(j + 1) * 4 + r] - Fh[j * 4 + r]) / R(dy);
+        }
+    }
+
+    // Switch samples of every face of one direction on the current (ghost-filled)
+    // state: x faces in [NY][NX+1] order, y faces in [NY+1][NX] order; per face
+    // 40 features ([field][10]) and 4 labels.
+    void samples(int dir, float* X, unsigned char* Y) const {
+        const int sx = g, ex = g + NX, sy = g, ey = g + NY;
+        std::vector<double> Fm((size_t)std::max(TX, TY) * 4), Fp(Fm.size());
+        const int lines = dir == 0 ? NY : NX;
+        for (int l = 0; l < lines; ++l) {
+            const int n = dir == 0 ? TX : TY;
+            for (int p = 0; p < n; ++p) {
+                const double* U = dir == 0 ? &Uh[idx(p, sy + l)] : &Uh[idx(sx + l, p)];
+                flux_split<double>(U, dir, cs.gamma, &Fm[p * 4], &Fp[p * 4]);
+            }
+            const int f0 = dir == 0 ? sx : sy, f1 = dir == 0 ? ex : ey;
+            for (int f = f0; f <= f1; ++f) {
+                double fp5[5][4], fm5[5][4], vp[5][4], vm[5][4];
+                for (int s = -2; s <= 2; ++s)
+                    for (int r = 0; r < 4; ++r) {
+                        fp5[s + 2][r] = Fp[(f + s - 1) * 4 + r];
+                        fm5[s + 2][r] = Fm[(f + s) * 4 + r];
+                    }
+                const double* Ul = dir == 0 ? &Uh[idx(f - 1, sy + l)] : &Uh[idx(sx + l, f - 1)];
+                const double* Ur = dir == 0 ? &Uh[idx(f, sy + l)] : &Uh[idx(sx + l, f)];
+                face_char(Ul, Ur, fp5, fm5, dir, cs.gamma, vp, vm);
+                const size_t q = dir == 0 ? (size_t)l * (NX + 1) + (f - f0) : (size_t)(f - f0) * NX + l;
+                face_samples(vp, vm, X + 40 * q, Y + 4 * q);
+            }
         }
     }
 
@@ -878,6 +1120,55 @@ int oracle_run(OracleHandle* h, int max_steps, double dt_fixed) {
 }
 
 double oracle_now(OracleHandle* h) { return h->s.now; }
+
+// ---- learned switch -------------------------------------------------------
+double oracle_linear5(double a, double b, double c, double d, double e) { return orc::linear5<double>(a, b, c, d, e); }
+
+void oracle_stencil_features(const double* w5, float* out5) { orc::stencil_features(w5, out5); }
+
+double oracle_weno_z_departure(const double* w5) {
+    return orc::weno_z_departure(w5[0], w5[1], w5[2], w5[3], w5[4]);
+}
+
+// Ghost fill, then features [faces][4][10] and labels [faces][4] of every face
+// of direction dir (x: [NY][NX+1] faces, y: [NY+1][NX] faces).
+void oracle_samples(OracleHandle* h, int dir, float* X, unsigned char* Y) {
+    h->s.fill_ghosts();
+    h->s.samples(dir, X, Y);
+}
+
+// Flags for the hybrid flux (arrays stay owned by the caller and must outlive
+// their use); null pointers restore the plain solver.
+void oracle_set_flags(OracleHandle* h, const unsigned char* fx, const unsigned char* fy) {
+    h->s.flags_x = fx;
+    h->s.flags_y = fy;
+}
+
+double oracle_mlp_loss_grad(const float* P, const float* X, const unsigned char* Y, int n, double pos_weight,
+                            double* G) {
+    double Pd[orc::P_N];
+    for (int q = 0; q < orc::P_N; ++q) Pd[q] = P[q];
+    return orc::mlp_loss_grad(Pd, X, Y, n, pos_weight, G);
+}
+
+// Same as oracle_mlp_loss_grad with double parameters (finite-difference checks).
+double oracle_mlp_loss_grad_d(const double* P, const float* X, const unsigned char* Y, int n, double pos_weight,
+                              double* G) {
+    return orc::mlp_loss_grad(P, X, Y, n, pos_weight, G);
+}
+
+void oracle_mlp_forward(const float* P, const float* X, int n, double* z) {
+    double Pd[orc::P_N];
+    for (int q = 0; q < orc::P_N; ++q) Pd[q] = P[q];
+    for (int s = 0; s < n; ++s) {
+        orc::MlpCache c;
+        z[s] = orc::mlp_forward(Pd, X + 10 * (size_t)s, c);
+    }
+}
+
+void oracle_mlp_infer_bf16(const float* P, const float* X, int n, float* z) {
+    for (int s = 0; s < n; ++s) z[s] = orc::mlp_infer_bf16(P, X + 10 * (size_t)s);
+}
 
 void oracle_accuracy(OracleHandle* h, double* out3) { h->s.accuracy(out3); }
 

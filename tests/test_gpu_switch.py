@@ -1,0 +1,244 @@
+"""The learned troubled-stencil switch on the GPU against the CPU oracle.
+
+Bars: harvested features and labels equal the oracle's bit for bit (float ==);
+a switched step equals the oracle's hybrid step with the same flags bit for
+bit; with every face flagged the hybrid kernels give the plain solver's state
+bit for bit; the bf16 MFMA inference matches the oracle's bf16 emulation to
+2e-2 (absolute, on logits of order 1-10; the MFMA summation order differs) and
+the FP64 forward to 0.15; the FP32 training gradient matches the oracle's FP64
+gradient to 1e-4 relative (norm) — the oracle's gradient is itself pinned to
+central differences in test_oracle_switch.py.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import wenofd as W
+from conftest import same
+
+pytestmark = pytest.mark.gpu
+
+QUAD_STRONG = np.array([1, 0, 0, 1000, 1, 0, 0, 0.01, 1, 0, 0, 0.01, 1, 0, 0, 0.01], float)
+
+
+def cfg(tc, sc, rk, cfl, nx, ny, **extra):
+    keys = dict(TESTCASE=tc, SCHEME=sc, RKMETH=rk, CFL="%.17g" % cfl, ELEMNUMX=nx, ELEMNUMY=ny, TEND=100)
+    keys.update(extra)
+    return W.config_text(**keys)
+
+
+def pair(tc, nx, ny, rk=2, cfl=0.5, seed=None, quad=None):
+    extra = {}
+    q = None
+    if seed is not None:
+        extra["QUAD_SEED"] = seed
+        q = O.seeded_quadrant(seed)
+    if quad is not None:
+        extra["QUAD_STATES"] = " ".join("%.17g" % v for v in quad)
+        q = np.asarray(quad, float).reshape(4, 4)
+    print(f"[gpu] switch case tc={tc} {nx}x{ny} rk={rk} seed={seed}", flush=True)
+    g = W.Solver(cfg(tc, 1, rk, cfl, nx, ny, **extra))
+    o = O.Solver(tc, 1, rk, cfl, nx, ny, t_end=100.0, quad=q)
+    return g, o
+
+
+def random_params(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    P = np.empty(W.N_PARAMS, np.float32)
+    P[0:320] = rng.uniform(-1, 1, 320) / np.sqrt(10) * scale
+    P[320:352] = rng.uniform(-1, 1, 32) / np.sqrt(10)
+    P[352:1376] = rng.uniform(-1, 1, 1024) / np.sqrt(32) * scale
+    P[1376:1408] = rng.uniform(-1, 1, 32) / np.sqrt(32)
+    P[1408:1441] = rng.uniform(-1, 1, 33) / np.sqrt(32) * scale
+    return P
+
+
+@pytest.mark.parametrize("tc,seed", [(5, 7), (4, None), (3, None)])
+def test_harvest_matches_oracle(tc, seed):
+    g, o = pair(tc, 22, 19, seed=seed)
+    for _ in range(2):  # a non-trivial state: two steps on both sides
+        dt = o.delta_t()
+        o.step(dt)
+        g.step(dt)
+    assert same(g.get_state(), o.get_state())
+    for d in (0, 1):
+        Xg, Yg = g.harvest(d)
+        Xo, Yo = o.samples(d)
+        assert same(Xg, Xo), f"features differ in direction {d}"
+        assert same(Yg, Yo)
+    g.close()
+
+
+@pytest.mark.parametrize("rk", [0, 1, 2])
+def test_all_flagged_equals_plain_and_all_linear_matches_oracle(rk):
+    a, o = pair(5, 20, 17, rk=rk, seed=3)
+    b = W.Solver(cfg(5, 1, rk, 0.5, 20, 17, QUAD_SEED=3))
+    c = W.Solver(cfg(5, 1, rk, 0.5, 20, 17, QUAD_SEED=3))
+    b.switch_set(W.SWITCH_WENO)
+    c.switch_set(W.SWITCH_LINEAR)
+    o.set_flags(np.zeros((17, 21), np.uint8), np.zeros((18, 20), np.uint8))
+    for _ in range(3):
+        a.step()
+        b.step()
+        dt = o.delta_t()
+        assert c.step() == dt
+        o.step(dt)
+    assert same(a.get_state(), b.get_state())
+    assert same(c.get_state(), o.get_state())
+    assert not same(a.get_state(), c.get_state())
+    for s in (a, b, c):
+        s.close()
+
+
+def test_mlp_inference_matches_oracle():
+    g, _ = pair(5, 40, 36, seed=11)
+    X = np.concatenate([g.harvest(0)[0].reshape(-1, 10), g.harvest(1)[0].reshape(-1, 10)])
+    rng = np.random.default_rng(0)
+    X = np.concatenate([X[rng.choice(len(X), 3000, replace=False)], rng.normal(size=(97, 10)).astype(np.float32)])
+    with W.Mlp(seed=1) as m:
+        P = random_params(2, scale=2.0)
+        m.params = P
+        assert same(m.params, P)
+        z = m.infer(X)
+    zb = O.mlp_infer_bf16(P, X)
+    z64 = O.mlp_forward(P, X)
+    assert np.abs(z - zb).max() < 2e-2, np.abs(z - zb).max()
+    assert np.abs(z - z64).max() < 0.15
+    assert np.std(z64) > 0.1  # the check is not vacuous
+    g.close()
+
+
+def test_classifier_logits_flags_and_dilation():
+    g, _ = pair(5, 45, 38, seed=5)
+    g.step()
+    P = random_params(3, scale=2.0)
+    with W.Mlp() as m:
+        m.params = P
+        Xx, _ = g.harvest(0)
+        Xy, _ = g.harvest(1)
+        zx = m.infer(Xx.reshape(-1, 10)).reshape(38, 46, 4)
+        zy = m.infer(Xy.reshape(-1, 10)).reshape(39, 45, 4)
+    thr = float(np.median(zx))
+    g.switch_set(W.SWITCH_MLP, P, thr)
+    fx, fy, lx, ly = g.switch_classify(dilate=False, logits=True)
+    assert same(lx, zx) and same(ly, zy)  # the same device arithmetic on the same features
+    assert same(fx, (zx > thr).any(axis=2)) and same(fy, (zy > thr).any(axis=2))
+    dx, dy, _, _ = g.switch_classify(dilate=True)
+
+    def dil(f):
+        p = np.pad(f, 1)
+        return np.max([p[1 + a:1 + a + f.shape[0], 1 + b:1 + b + f.shape[1]] for a in (-1, 0, 1) for b in (-1, 0, 1)],
+                      axis=0)
+
+    assert same(dx, dil(fx)) and same(dy, dil(fy))
+    assert 0 < fx.mean() < 1
+    g.close()
+
+
+@pytest.mark.parametrize("rk", [1, 2])
+def test_switched_steps_match_oracle_with_the_same_flags(rk):
+    g, o = pair(5, 34, 29, rk=rk, seed=9)
+    P = random_params(4, scale=2.0)
+    g.switch_set(W.SWITCH_MLP, P, 0.0)
+    _, _, lx, ly = g.switch_classify(dilate=False, logits=True)
+    # a threshold that leaves about 10% of the faces flagged before dilation
+    thr = float(np.quantile(np.concatenate([lx.max(axis=2).ravel(), ly.max(axis=2).ravel()]), 0.9))
+    g.switch_set(W.SWITCH_MLP, P, thr)
+    mixed = []
+    for _ in range(3):
+        dt = g.step()
+        fx, fy = g.switch_flags()
+        mixed.append(0 < fx.mean() < 1 and 0 < fy.mean() < 1)
+        o.set_flags(fx, fy)
+        assert o.delta_t() == dt
+        o.step(dt)
+        assert same(g.get_state(), o.get_state())
+    assert all(mixed)
+    g.close()
+
+
+def nan_time_step(U, gamma=1.4):
+    """Whether any point's local CFL step is NaN (the library's NaN test; the
+    reference's own minimum loop can lose a NaN that a finite value follows)."""
+    with np.errstate(all="ignore"):
+        rho = U[..., 0]
+        u, v = U[..., 1] / rho, U[..., 2] / rho
+        p = (gamma - 1.0) * (U[..., 3] - (0.5 * rho) * (u * u + v * v))
+        c = np.sqrt((gamma * p) / rho)
+        return bool(np.isnan(np.abs(u) + c).any() or np.isnan(np.abs(v) + c).any())
+
+
+def test_fallback_redoes_a_failed_switched_step_with_the_plain_solver():
+    n = 24
+    g, o = pair(5, n, n, cfl=0.6, quad=QUAD_STRONG)
+    g.switch_set(W.SWITCH_LINEAR)
+    zeros = (np.zeros((n, n + 1), np.uint8), np.zeros((n + 1, n), np.uint8))
+    o.set_flags(*zeros)
+    prev = None
+    fallbacks = 0
+    for _ in range(6):
+        dt = o.delta_t()
+        if nan_time_step(o.get_state()):  # the step before left a NaN time step: redo it without the switch
+            o.set_state(prev[0])
+            o.set_flags(None, None)
+            o.step(prev[1])
+            o.set_flags(*zeros)
+            fallbacks += 1
+            dt = o.delta_t()
+            assert not nan_time_step(o.get_state())
+        prev = (o.get_state(), dt)
+        o.step(dt)
+        assert g.step() == dt
+        assert same(g.get_state(), o.get_state())
+    assert fallbacks >= 1 and g.switch_fallbacks() == fallbacks
+    g.close()
+
+
+def test_training_gradient_matches_oracle():
+    g, _ = pair(5, 40, 40, seed=13)
+    g.step()
+    with W.Mlp(seed=3) as m:
+        m.harvest(g, keep_pos=1.0, keep_neg=0.3, seed=1)
+        n, pos = m.size()
+        assert n > 1000 and 0 < pos < n
+        X, Y = m.samples()
+        rng = np.random.default_rng(1)
+        idx = rng.choice(n, 700, replace=False).astype(np.int32)
+        loss, G = m.grad(idx, pos_weight=2.5)
+        lo, Go = O.mlp_loss_grad(m.params, X[idx], Y[idx], pos_weight=2.5)
+    assert abs(loss - lo) <= 1e-5 * max(1.0, abs(lo))
+    assert np.linalg.norm(G - Go) <= 1e-4 * np.linalg.norm(Go)
+    assert np.abs(G - Go).max() <= 1e-4 * np.abs(Go).max()
+    g.close()
+
+
+def test_harvest_keeps_a_deterministic_subset():
+    g, o = pair(5, 30, 26, seed=21)
+    Xall = np.concatenate([g.harvest(0)[0].reshape(-1, 10), g.harvest(1)[0].reshape(-1, 10)])
+    Yall = np.concatenate([g.harvest(0)[1].reshape(-1), g.harvest(1)[1].reshape(-1)])
+    with W.Mlp() as a, W.Mlp() as b:
+        assert a.harvest(g, 1.0, 1.0, seed=5) == len(Yall)
+        Xa, Ya = a.samples()
+        assert same(Xa, Xall) and same(Ya, Yall)  # keep-all preserves the face order
+        na = b.harvest(g, 0.5, 0.1, seed=7)
+        Xb, Yb = b.samples()
+        b.clear()
+        assert b.harvest(g, 0.5, 0.1, seed=7) == na
+        assert same(b.samples()[0], Xb)
+        assert 0 < na < len(Yall)
+    g.close()
+
+
+def test_training_is_deterministic_and_lowers_the_loss():
+    g, _ = pair(5, 48, 48, seed=17)
+    for _ in range(3):
+        g.step()
+    runs = []
+    for _ in range(2):
+        with W.Mlp(seed=4) as m:
+            m.harvest(g, keep_pos=1.0, keep_neg=0.5, seed=2)
+            losses = m.train(epochs=4, batch=1024, lr=3e-3, pos_weight=2.0, seed=9)
+            runs.append((losses, m.params))
+    assert same(runs[0][0], runs[1][0]) and same(runs[0][1], runs[1][1])
+    assert runs[0][0][-1] < runs[0][0][0]
+    g.close()
