@@ -17,6 +17,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "fd-weno-eulerequation_amd")
 sys.path.insert(0, PKG)
@@ -127,6 +129,61 @@ def batched(W, steps, seq_steps, seed=0):
             "sequential_value": seq, "sequential_steps": seq_steps, "speedup_vs_sequential": value / seq}
 
 
+def hybrid(W, n, seed, t_end, truth):
+    """Learned switch against plain WENO-Z on an unseen seeded four-quadrant
+    problem (seed outside the training seeds 0..255), RK3 CFL 0.6 to t_end,
+    both timed over the whole run. With truth, both are also compared with
+    plain WENO-Z on the nested (2n-1)^2 grid sampled at even indices."""
+    P, thr = W.load_switch()
+    cfg = W.config_text(TESTCASE=5, SCHEME=1, RKMETH=2, CFL=0.6, ELEMNUMX=n, ELEMNUMY=n, QUAD_SEED=seed,
+                        TEND="%.17g" % t_end)
+    res = {}
+    for name, mode in (("plain", W.SWITCH_OFF), ("hybrid", W.SWITCH_MLP)):
+        log(f"hybrid comparison: {name} {n}x{n} seed {seed} to t={t_end}")
+        s = W.Solver(cfg)
+        if mode:
+            s.switch_set(mode, P, thr)
+        s.synchronize()
+        t0 = time.perf_counter()
+        steps = s.advance(0)
+        s.synchronize()
+        el = time.perf_counter() - t0
+        r = {"steps": steps, "seconds": el, "ms_per_step": el * 1e3 / steps, "value": n * n * steps / el,
+             "rho": s.get_state()[..., 0]}
+        if mode:
+            fx, fy = s.switch_flags()
+            rx, ry, _, _ = s.switch_classify(dilate=False)
+            r["flags_dilated"] = float((fx.sum() + fy.sum()) / (fx.size + fy.size))
+            r["flags_raw"] = float((rx.sum() + ry.sum()) / (rx.size + ry.size))
+            r["fallbacks"] = s.switch_fallbacks()
+        s.close()
+        res[name] = r
+    d = np.abs(res["hybrid"]["rho"] - res["plain"]["rho"])
+    out = {"workload": f"unseen seeded four-quadrant problem (seed {seed}) {n}x{n}, WENO-Z RK3 CFL 0.6, t_end {t_end}",
+           "threshold": thr, "plain_value": res["plain"]["value"], "hybrid_value": res["hybrid"]["value"],
+           "speedup": res["hybrid"]["value"] / res["plain"]["value"],
+           "plain_ms_per_step": res["plain"]["ms_per_step"], "hybrid_ms_per_step": res["hybrid"]["ms_per_step"],
+           "steps": [res["plain"]["steps"], res["hybrid"]["steps"]],
+           "flag_rate_final_raw": res["hybrid"]["flags_raw"], "flag_rate_final_dilated": res["hybrid"]["flags_dilated"],
+           "fallbacks": res["hybrid"]["fallbacks"],
+           "rho_delta_vs_plain": {"L1": float(d.mean()), "Linf": float(d.max())}}
+    if truth:
+        m = 2 * n - 1
+        log(f"hybrid comparison: truth {m}x{m}")
+        cfg_t = W.config_text(TESTCASE=5, SCHEME=1, RKMETH=2, CFL=0.6, ELEMNUMX=m, ELEMNUMY=m, QUAD_SEED=seed,
+                              TEND="%.17g" % t_end)
+        with W.Solver(cfg_t) as s:
+            s.advance(0)
+            rt = s.get_state()[::2, ::2, 0]
+        ep = np.abs(res["plain"]["rho"] - rt)
+        eh = np.abs(res["hybrid"]["rho"] - rt)
+        out["rho_error_vs_truth"] = {"truth": f"plain WENO-Z {m}x{m} at even indices",
+                                     "plain": {"L1": float(ep.mean()), "Linf": float(ep.max())},
+                                     "hybrid": {"L1": float(eh.mean()), "Linf": float(eh.max())},
+                                     "L1_ratio": float(eh.mean() / ep.mean())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +195,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for cpu_baseline (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=200, help="steps of the 64-solve batch (0: skip)")
     ap.add_argument("--seq-steps", type=int, default=20, help="steps per solve of the sequential comparison")
+    ap.add_argument("--hybrid-size", type=int, default=2048, help="grid of the learned-switch comparison (0: skip)")
+    ap.add_argument("--hybrid-seed", type=int, default=1000)
+    ap.add_argument("--hybrid-tend", type=float, default=0.25)
+    ap.add_argument("--hybrid-truth", action="store_true", help="also compare with plain WENO-Z on the nested grid")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this solver runs one whole domain on one GPU; --gpus must be 1", file=sys.stderr)
@@ -207,6 +268,8 @@ def main():
     }
     if args.batch_steps > 0:
         rec["batched"] = batched(W, args.batch_steps, args.seq_steps)
+    if args.hybrid_size > 0:
+        rec["hybrid"] = hybrid(W, args.hybrid_size, args.hybrid_seed, args.hybrid_tend, args.hybrid_truth)
     if args.cpu_steps > 0:
         log(f"cpu baseline: {args.cpu_steps} oracle steps at {n}x{n}")
         rec["cpu_baseline"] = cpu_baseline(n, n, sc, args.cfl, args.cpu_steps)

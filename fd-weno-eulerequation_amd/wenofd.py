@@ -19,6 +19,7 @@ K_GHOST, K_FLUX_X, K_FLUX_Y, K_SWITCH, K_DT, K_COUNT = range(6)
 KERNEL_NAMES = ("ghost_fill", "flux_x_rk", "flux_y", "switch", "delta_t")
 SWITCH_OFF, SWITCH_MLP, SWITCH_LINEAR, SWITCH_WENO = range(4)
 N_PARAMS = 1441
+SWITCH_WEIGHTS = os.path.join(HERE, "switch", "switch_mlp.npz")
 
 # Exported symbols declared in include/wenofd_amd.h.
 EXPORTS = (
@@ -350,6 +351,12 @@ class Solver:
         Y = np.empty((nf, 4), np.uint8)
         _check(lib().wenofd_harvest(self.h, int(direction), _fptr(X), _u8ptr(Y)))
         return X, Y
+
+
+def load_switch(path=SWITCH_WEIGHTS):
+    """(params float32[1441], threshold) of a trained switch (switch_train.py)."""
+    with np.load(path) as z:
+        return np.asarray(z["params"], np.float32), float(z["threshold"])
 
 
 class Mlp:

@@ -242,3 +242,57 @@ def test_training_is_deterministic_and_lowers_the_loss():
     assert same(runs[0][0], runs[1][0]) and same(runs[0][1], runs[1][1])
     assert runs[0][0][-1] < runs[0][0][0]
     g.close()
+
+
+def test_ablation_trained_switch_beats_random_flags_at_the_same_rate():
+    """What the trained network gains over the method without it: at the same
+    fraction of WENO faces, flags placed by the network keep the solution much
+    closer to plain WENO-Z than flags placed at random, and the cheap flux
+    alone (no network) cannot run the problem without the plain fallback."""
+    n, seed, t_end = 128, 1003, 0.1
+    P, thr = W.load_switch()
+    cfg_ = cfg(5, 1, 2, 0.6, n, n, QUAD_SEED=seed, TEND=t_end)
+    print(f"[gpu] ablation {n}^2 seed {seed}", flush=True)
+    with W.Solver(cfg_) as s:
+        s.advance(0)
+        plain = s.get_state()[..., 0]
+    with W.Solver(cfg_) as s:
+        s.switch_set(W.SWITCH_MLP, P, thr)
+        steps = s.advance(0)
+        trained = s.get_state()[..., 0]
+        assert s.switch_fallbacks() == 0
+    # random flags at the trained run's final flag rate, drawn every step
+    rng = np.random.default_rng(0)
+    with W.Solver(cfg_) as s:
+        s.switch_set(W.SWITCH_MLP, P, thr)
+        s.advance(0)
+        fx, fy = s.switch_flags()
+        rate = (fx.sum() + fy.sum()) / (fx.size + fy.size)
+    o = O.Solver(5, 1, 2, 0.6, n, n, t_end=t_end, quad=O.seeded_quadrant(seed))
+    while abs(o.now() - t_end) > 1e-9:
+        o.set_flags((rng.uniform(size=fx.shape) < rate).astype(np.uint8),
+                    (rng.uniform(size=fy.shape) < rate).astype(np.uint8))
+        dt = o.delta_t()
+        if nan_time_step(o.get_state()):
+            break
+        o.step(dt)
+    rand = o.get_state()[..., 0]
+    with W.Solver(cfg_) as s:
+        s.switch_set(W.SWITCH_LINEAR)
+        try:
+            s.advance(0)
+            linear_failed = False
+        except W.WenofdError as e:  # even the plain redo of a step cannot recover the state
+            assert e.code == W.E_NAN_DT
+            linear_failed = True
+        linear_fallbacks = s.switch_fallbacks()
+        linear = s.get_state()[..., 0]
+    e_trained = np.abs(trained - plain).mean()
+    e_rand = np.abs(rand - plain).mean()
+    e_lin = np.abs(linear - plain).mean()
+    print(f"[gpu] ablation: steps {steps}, flag rate {rate:.3f}, L1 delta trained {e_trained:.3e} "
+          f"random {e_rand:.3e} linear {e_lin:.3e} (linear fallbacks {linear_fallbacks}, failed {linear_failed})",
+          flush=True)
+    assert 0.05 < rate < 0.95
+    assert e_trained * 20 < e_rand
+    assert linear_failed or linear_fallbacks > 0 or e_trained * 20 < e_lin
