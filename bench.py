@@ -74,6 +74,34 @@ def measured_traffic(kernel, n):
     return None
 
 
+def measured_valu(kernel, n):
+    """VALU wave-instructions per launch from the committed counter summary
+    (SQ_INSTS_VALU), or None when not measured for n."""
+    path = os.path.join(ROOT, "profiles", "valu_vortex2048_wenoz_rk3.md")
+    if n != 2048 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("valu_2048:"):
+            vals = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
+            return int(vals[kernel]) if kernel in vals else None
+    return None
+
+
+# FP64 VALU issue capacity: 256 CUs x 4 SIMDs, one wave64 FP64 instruction per 4 cycles, 2.4 GHz.
+VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
+
+
+def valu_issue(kernel, n, avg_us):
+    """Measured VALU wave-instructions per launch over this run's duration,
+    against the chip's FP64 issue capacity."""
+    w = measured_valu(kernel, n)
+    if w is None:
+        return None
+    achieved = w / (avg_us * 1e-6)
+    return {"wave_instructions_per_launch": w, "achieved": achieved, "peak": VALU_PEAK_WAVE_INSTR,
+            "unit": "wave-instructions/s", "frac": achieved / VALU_PEAK_WAVE_INSTR}
+
+
 def cpu_baseline(nx, ny, scheme, cfl, steps):
     """The CPU oracle (a from-scratch serial restatement of the reference's
     arithmetic) timed on this host over a bounded sample of the same workload."""
@@ -262,6 +290,7 @@ def main():
             "frac": dk["tflops"] / FP64_PEAK_TFLOPS,
             "traffic": measured_traffic(dom, n) if args.scheme == "Z" else None,
             "hbm": {"achieved": dk["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dk["gbs"] / HBM_PEAK_GBS},
+            "valu_issue": valu_issue(dom, n, dk["avg_us"]) if args.scheme == "Z" else None,
             "step_fp64_tflops": step_flops / (el / args.steps) / 1e12,
             "kernels": kernels,
         },
