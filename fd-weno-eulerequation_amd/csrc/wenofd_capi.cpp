@@ -85,6 +85,8 @@ struct wenofd_solver {
     // dilated flags per direction (x faces [ny][nx+1], y faces [ny+1][nx]).
     int sw_mode = WENOFD_SWITCH_OFF;
     float sw_thr = 0.0f;
+    int sw_every = 1;           // classify every sw_every steps, dilation radius sw_every
+    long long sw_since = 0;     // switched steps since the last classification (0: classify now)
     float* sw_params = nullptr;
     unsigned char* fl_raw_x = nullptr;
     unsigned char* fl_raw_y = nullptr;
@@ -179,19 +181,25 @@ int stage(wenofd_solver* h, double* in, const double* un, double* out, const RkC
 }
 
 // Ghost fill of U, then (MLP mode) the flags of every face from the network,
-// dilated by one face in both directions of the face grid: the flags of a
-// step are computed once from its initial state and serve all its stages.
-int classify(wenofd_solver* h, float* logits_x = nullptr, float* logits_y = nullptr, bool dilate = true) {
+// dilated by sw_every faces in both directions of the face grid: the flags are
+// computed from the initial state of every sw_every-th step and serve all
+// stages of that many steps (a wave moves less than one cell per step at
+// CFL <= 1).
+int classify(wenofd_solver* h, float* logits_x = nullptr, float* logits_y = nullptr, bool dilate = true,
+             bool force = true) {
     int rc;
     if ((rc = timed(h, WENOFD_K_GHOST, [&] { return launch_ghost_fill(h->U, h->special, h->g, h->bc, h->stream); })))
         return rc;
     if (h->sw_mode != WENOFD_SWITCH_MLP) return 0;
+    if (!force && h->sw_since > 0 && h->sw_since < h->sw_every) return 0;
+    h->sw_since = 0;
+    const int rad = h->sw_every;
     return timed(h, WENOFD_K_SWITCH, [&] {
         hipError_t e = launch_classify(0, h->U, h->g, h->sw_params, h->sw_thr, h->fl_raw_x, logits_x, h->stream);
         if (e == hipSuccess)
             e = launch_classify(1, h->U, h->g, h->sw_params, h->sw_thr, h->fl_raw_y, logits_y, h->stream);
-        if (e == hipSuccess && dilate) e = launch_dilate(h->fl_raw_x, h->fl_x, h->g.ny, h->g.nx + 1, h->stream);
-        if (e == hipSuccess && dilate) e = launch_dilate(h->fl_raw_y, h->fl_y, h->g.ny + 1, h->g.nx, h->stream);
+        if (e == hipSuccess && dilate) e = launch_dilate(h->fl_raw_x, h->fl_x, h->g.ny, h->g.nx + 1, rad, h->stream);
+        if (e == hipSuccess && dilate) e = launch_dilate(h->fl_raw_y, h->fl_y, h->g.ny + 1, h->g.nx, rad, h->stream);
         return e;
     });
 }
@@ -203,7 +211,8 @@ int run_rk_switched(wenofd_solver* h, int rkmeth, double dt) {
     int rc;
     h->dt_parts = 0;
     h->prev_slot = nullptr;
-    if ((rc = classify(h))) return rc;
+    if ((rc = classify(h, nullptr, nullptr, true, false))) return rc;
+    h->sw_since += 1;
     if (rkmeth == 0) {
         RkCoef k{kRk1, grav, 0.0, 1.0, dt};
         if ((rc = stage(h, h->U, nullptr, h->Ua, k, nullptr, true, false))) return rc;
@@ -309,6 +318,7 @@ int upload_initial(wenofd_solver* h) {
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->dt_parts = 0;
     h->prev_slot = nullptr;
+    h->sw_since = 0;
     h->now = 0;
     h->steps = 0;
     return 0;
@@ -565,6 +575,7 @@ int wenofd_set_state(wenofd_solver* h, const double* host, int layout) {
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->dt_parts = 0;
     h->prev_slot = nullptr;
+    h->sw_since = 0;
     return 0;
 }
 
@@ -589,6 +600,7 @@ int wenofd_step(wenofd_solver* h, double dt_fixed, double* dt_used) {
         h->now += h->prev_dt;
         h->steps += 1;
         h->fallbacks += 1;
+        h->sw_since = 0;  // the next switched step classifies afresh
         if ((rc = delta_t(h, &dt))) return rc;
     }
     if ((rc = run_rk(h, h->opt.rkmeth, dt))) return rc;
@@ -813,7 +825,16 @@ int wenofd_switch_set(wenofd_solver* h, int mode, const float* params, float thr
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->sw_mode = mode;
     h->sw_thr = threshold;
+    h->sw_since = 0;
     h->prev_slot = nullptr;
+    return 0;
+}
+
+int wenofd_switch_interval(wenofd_solver* h, int steps) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    if (steps < 1 || steps > 16) return fail(WENOFD_E_ARG, "the classification interval must be in [1, 16]");
+    h->sw_every = steps;
+    h->sw_since = 0;
     return 0;
 }
 

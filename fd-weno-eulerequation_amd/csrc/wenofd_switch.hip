@@ -53,22 +53,25 @@ __device__ __forceinline__ void mlp_load(const float* __restrict__ P, MlpRegs& R
     R.b3 = P[kOffB3];
 }
 
+// The biases enter as the accumulators' initial values (the lane's C rows are
+// exactly the units whose biases it holds), so no separate bias additions.
 __device__ __forceinline__ float mlp_group(const MlpRegs& R, bf16x4 x) {
-    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
     bf16x4 h[2];
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-        const f32x4 c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w1[m], x, zero, 0, 0, 0);
+        const f32x4 b = {R.b1[m][0], R.b1[m][1], R.b1[m][2], R.b1[m][3]};
+        const f32x4 c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w1[m], x, b, 0, 0, 0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) h[m][j] = (__bf16)gelu_fast(c[j] + R.b1[m][j]);
+        for (int j = 0; j < 4; ++j) h[m][j] = (__bf16)gelu_fast(c[j]);
     }
     float z = 0.0f;
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-        f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w2[m][0], h[0], zero, 0, 0, 0);
+        const f32x4 b = {R.b2[m][0], R.b2[m][1], R.b2[m][2], R.b2[m][3]};
+        f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w2[m][0], h[0], b, 0, 0, 0);
         d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w2[m][1], h[1], d, 0, 0, 0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) z += R.w3[m][j] * gelu_fast(d[j] + R.b2[m][j]);
+        for (int j = 0; j < 4; ++j) z += R.w3[m][j] * gelu_fast(d[j]);
     }
     z += __shfl_xor(z, 16, 64);
     z += __shfl_xor(z, 32, 64);
@@ -252,17 +255,17 @@ __global__ __launch_bounds__(kSwT) void k_switch_faces(const double* __restrict_
     }
 }
 
-// out[r][c] = OR of in over the 3 x 3 neighbourhood (rows x cols grid of faces).
+// out[r][c] = OR of in over the (2 rad + 1)^2 neighbourhood (rows x cols grid of faces).
 __global__ __launch_bounds__(256) void k_dilate(const unsigned char* __restrict__ in, unsigned char* __restrict__ out,
-                                                int rows, int cols) {
+                                                int rows, int cols, int rad) {
     const long k = (long)blockIdx.x * 256 + threadIdx.x;
     if (k >= (long)rows * cols) return;
     const int r = (int)(k / cols), c = (int)(k % cols);
     unsigned char v = 0;
-    for (int dr = -1; dr <= 1; ++dr) {
+    for (int dr = -rad; dr <= rad; ++dr) {
         const int rr = r + dr;
         if (rr < 0 || rr >= rows) continue;
-        for (int dc = -1; dc <= 1; ++dc) {
+        for (int dc = -rad; dc <= rad; ++dc) {
             const int cc = c + dc;
             if (cc >= 0 && cc < cols) v |= in[(size_t)rr * cols + cc];
         }
@@ -499,9 +502,9 @@ hipError_t launch_harvest(int dir, const double* U, const Geom& g, float* X, uns
     return hipGetLastError();
 }
 
-hipError_t launch_dilate(const unsigned char* in, unsigned char* out, int rows, int cols, hipStream_t s) {
+hipError_t launch_dilate(const unsigned char* in, unsigned char* out, int rows, int cols, int rad, hipStream_t s) {
     const long n = (long)rows * cols;
-    hipLaunchKernelGGL(k_dilate, dim3((n + 255) / 256), dim3(256), 0, s, in, out, rows, cols);
+    hipLaunchKernelGGL(k_dilate, dim3((n + 255) / 256), dim3(256), 0, s, in, out, rows, cols, rad);
     return hipGetLastError();
 }
 

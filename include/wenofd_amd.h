@@ -173,6 +173,9 @@ int wenofd_switch_set(wenofd_solver* h, int mode, const float* params, float thr
  * [NY+1][NX]. */
 int wenofd_switch_classify(wenofd_solver* h, int dilate, unsigned char* flags_x, unsigned char* flags_y,
                            float* logits_x, float* logits_y);
+/* Classify every `steps` steps (1..16, default 1) and dilate the flags by
+ * `steps` faces: a wave moves less than one cell per step at CFL <= 1. */
+int wenofd_switch_interval(wenofd_solver* h, int steps);
 /* The (dilated) flags the last switched step used. */
 int wenofd_switch_flags(wenofd_solver* h, unsigned char* flags_x, unsigned char* flags_y);
 /* Number of switched steps the plain solver had to redo. */

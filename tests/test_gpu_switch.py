@@ -296,3 +296,31 @@ def test_ablation_trained_switch_beats_random_flags_at_the_same_rate():
     assert 0.05 < rate < 0.95
     assert e_trained * 20 < e_rand
     assert linear_failed or linear_fallbacks > 0 or e_trained * 20 < e_lin
+
+
+def test_classification_interval_reuses_dilated_flags():
+    g, o = pair(5, 40, 34, seed=19)
+    P, thr = W.load_switch()
+    g.switch_set(W.SWITCH_MLP, P, thr)
+    g.switch_interval(2)
+    rx, ry, _, _ = g.switch_classify(dilate=False)
+
+    def dil(f, r):
+        p = np.pad(f, r)
+        return np.max([p[r + a:r + a + f.shape[0], r + b:r + b + f.shape[1]]
+                       for a in range(-r, r + 1) for b in range(-r, r + 1)], axis=0)
+
+    seen = []
+    for k in range(4):
+        dt = g.step()
+        fx, fy = g.switch_flags()
+        if k == 0:
+            assert same(fx, dil(rx, 2)) and same(fy, dil(ry, 2))
+        seen.append((fx, fy))
+        o.set_flags(fx, fy)
+        assert o.delta_t() == dt
+        o.step(dt)
+        assert same(g.get_state(), o.get_state())
+    assert same(seen[1][0], seen[0][0]) and same(seen[3][0], seen[2][0])  # reused for a second step
+    assert not same(seen[2][0], seen[1][0])  # reclassified
+    g.close()
