@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "wenofd_amd.h"
+#include "wenofd_devloop.hpp"
 #include "wenofd_host.hpp"
 #include "wenofd_kernels.hpp"
 
@@ -53,13 +54,7 @@ struct wenofd_batch {
     int* active_d = nullptr;
     int* active_h = nullptr;  // pinned
     size_t u_total = 0, fy_total = 0, sp_total = 0, part_cap = 0;
-    Tiles tiles{};
-    TileRef* tab_ghost = nullptr;
-    TileRef* tab_y = nullptr;
-    TileRef* tab_x = nullptr;
-    int n_ghost = 0, n_y = 0, n_x = 0;
-    int graph_steps = 16;  // steps per graph replay; 0: plain launches
-    hipGraphExec_t exec = nullptr;
+    DevLoop loop;
     double last_ms_per_step = 0;
 
     BatchView view() const {
@@ -81,19 +76,88 @@ namespace {
 
 int fail(int code, const std::string& msg) { return set_last_error(code, msg); }
 
-void drop_graph(wenofd_batch* b) {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    b->exec = nullptr;
+int check_launch(hipError_t e) {
+    return e == hipSuccess ? 0 : fail(WENOFD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
 }
 
-// Block -> (solve, tile) tables of the three per-stage launches for the current
-// tile shape, and the per-solve layout of the dt partials.
-int build_tables(wenofd_batch* b) {
+// One stage of every active solve: ghost fill of `in`, y faces, x faces fused
+// with the rhs and the update into `out` (Un = buffer 0).
+int stage(DevLoop& L, const BatchView& v, int in, int out, int mode, double a, double bc, double c, bool last,
+          hipStream_t s) {
+    int rc;
+    if ((rc = check_launch(launch_batch_ghost(v, in, L.tab_ghost, L.n_ghost, s)))) return rc;
+    if ((rc = check_launch(launch_batch_flux_y(L.scheme, L.tiles, v, in, L.tab_y, L.n_y, s)))) return rc;
+    return check_launch(
+        launch_batch_flux_x_rk(L.scheme, L.tiles, v, in, out, mode, a, bc, c, last, L.tab_x, L.n_x, s));
+}
+
+// One iteration of the run() loop for every solve (src/WENOFD.cpp:213-239):
+// calculateDeltaT, RunRK<RKMETH>, now += dt. The RK coefficients are those of
+// src/WENOFD.cpp:301-396 (cdt = c * dt formed in the kernel).
+int enqueue_step(DevLoop& L, const BatchView& v, hipStream_t s) {
+    int rc;
+    if ((rc = check_launch(launch_batch_dt(v, s)))) return rc;
+    if (L.rkmeth == 0) {
+        if ((rc = stage(L, v, 0, 1, kRk1, 0.0, 1.0, 1.0, true, s))) return rc;
+        if ((rc = check_launch(launch_batch_copy(L.tiles, v, 1, 0, L.tab_x, L.n_x, s)))) return rc;
+    } else if (L.rkmeth == 1) {
+        if ((rc = stage(L, v, 0, 1, kRkStage, 1.0, 0.0, 1.0, false, s))) return rc;
+        if ((rc = stage(L, v, 1, 0, kRkStage, 0.5, 0.5, 0.5, true, s))) return rc;
+    } else {
+        if ((rc = stage(L, v, 0, 1, kRkStage, 1.0, 0.0, 1.0, false, s))) return rc;
+        if ((rc = stage(L, v, 1, 2, kRkStage, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, false, s))) return rc;
+        if ((rc = stage(L, v, 2, 0, kRkStage, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, true, s))) return rc;
+    }
+    return check_launch(launch_batch_advance(v, false, s));
+}
+
+int ensure_graph(DevLoop& L, const BatchView& v, hipStream_t s) {
+    if (L.exec && (L.graph_U[0] != v.U[0] || L.graph_U[1] != v.U[1] || L.graph_U[2] != v.U[2]))
+        devloop_drop_graph(L);  // the graph holds the buffer addresses it was captured with
+    if (L.exec || L.graph_steps <= 0) return 0;
+    BHIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int k = 0; k < L.graph_steps && !rc; ++k) rc = enqueue_step(L, v, s);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(WENOFD_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiateWithFlags(&L.exec, graph, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        L.exec = nullptr;
+        return fail(WENOFD_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+    }
+    for (int k = 0; k < 3; ++k) L.graph_U[k] = v.U[k];
+    return 0;
+}
+
+}  // namespace
+
+namespace wenofd {
+
+void devloop_drop_graph(DevLoop& L) {
+    if (L.exec) (void)hipGraphExecDestroy(L.exec);
+    L.exec = nullptr;
+}
+
+void devloop_free(DevLoop& L) {
+    devloop_drop_graph(L);
+    (void)hipFree(L.tab_ghost);
+    (void)hipFree(L.tab_y);
+    (void)hipFree(L.tab_x);
+    L.tab_ghost = L.tab_y = L.tab_x = nullptr;
+}
+
+int devloop_tables(DevLoop& L, std::vector<SolveDesc>& desc, SolveDesc* desc_d, size_t part_cap) {
     std::vector<TileRef> tg, ty, tx;
     long long part_off = 0;
-    for (int s = 0; s < b->nsolves; ++s) {
-        SolveDesc& d = b->desc[s];
-        const long ng = ghost_blocks(d.g), ny = flux_y_blocks(d.g, b->tiles), nx = flux_x_rk_blocks(d.g, b->tiles);
+    for (int s = 0; s < (int)desc.size(); ++s) {
+        SolveDesc& d = desc[s];
+        const long ng = ghost_blocks(d.g), ny = flux_y_blocks(d.g, L.tiles), nx = flux_x_rk_blocks(d.g, L.tiles);
         for (long k = 0; k < ng; ++k) tg.push_back({s, (int)k});
         for (long k = 0; k < ny; ++k) ty.push_back({s, (int)k});
         for (long k = 0; k < nx; ++k) tx.push_back({s, (int)k});
@@ -101,101 +165,49 @@ int build_tables(wenofd_batch* b) {
         d.nparts = (int)nx;
         part_off += nx;
     }
-    if ((size_t)part_off > b->part_cap) return fail(WENOFD_E_ARG, "dt partial buffer too small for the tile shape");
-    drop_graph(b);
-    (void)hipFree(b->tab_ghost);
-    (void)hipFree(b->tab_y);
-    (void)hipFree(b->tab_x);
-    b->tab_ghost = b->tab_y = b->tab_x = nullptr;
-    BHIP_TRY(hipMalloc(&b->tab_ghost, tg.size() * sizeof(TileRef)));
-    BHIP_TRY(hipMalloc(&b->tab_y, ty.size() * sizeof(TileRef)));
-    BHIP_TRY(hipMalloc(&b->tab_x, tx.size() * sizeof(TileRef)));
-    BHIP_TRY(hipMemcpy(b->tab_ghost, tg.data(), tg.size() * sizeof(TileRef), hipMemcpyHostToDevice));
-    BHIP_TRY(hipMemcpy(b->tab_y, ty.data(), ty.size() * sizeof(TileRef), hipMemcpyHostToDevice));
-    BHIP_TRY(hipMemcpy(b->tab_x, tx.data(), tx.size() * sizeof(TileRef), hipMemcpyHostToDevice));
-    BHIP_TRY(hipMemcpy(b->desc_d, b->desc.data(), b->desc.size() * sizeof(SolveDesc), hipMemcpyHostToDevice));
-    b->n_ghost = (int)tg.size();
-    b->n_y = (int)ty.size();
-    b->n_x = (int)tx.size();
+    if ((size_t)part_off > part_cap) return fail(WENOFD_E_ARG, "dt partial buffer too small for the tile shape");
+    devloop_free(L);
+    BHIP_TRY(hipMalloc(&L.tab_ghost, tg.size() * sizeof(TileRef)));
+    BHIP_TRY(hipMalloc(&L.tab_y, ty.size() * sizeof(TileRef)));
+    BHIP_TRY(hipMalloc(&L.tab_x, tx.size() * sizeof(TileRef)));
+    BHIP_TRY(hipMemcpy(L.tab_ghost, tg.data(), tg.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+    BHIP_TRY(hipMemcpy(L.tab_y, ty.data(), ty.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+    BHIP_TRY(hipMemcpy(L.tab_x, tx.data(), tx.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+    BHIP_TRY(hipMemcpy(desc_d, desc.data(), desc.size() * sizeof(SolveDesc), hipMemcpyHostToDevice));
+    L.n_ghost = (int)tg.size();
+    L.n_y = (int)ty.size();
+    L.n_x = (int)tx.size();
     return 0;
 }
 
-// The CFL minimum of every solve's current state into its partials, and the
-// count of running solves: the state the first step of a run starts from.
-int prime(wenofd_batch* b) {
-    const BatchView v = b->view();
-    BHIP_TRY(launch_batch_dt_partial(v, b->stream));
-    BHIP_TRY(launch_batch_advance(v, true, b->stream));
+int devloop_prime(const BatchView& v, hipStream_t s) {
+    BHIP_TRY(launch_batch_dt_partial(v, s));
+    BHIP_TRY(launch_batch_advance(v, true, s));
     return 0;
 }
 
-int check_launch(hipError_t e) {
-    return e == hipSuccess ? 0 : fail(WENOFD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
-}
-
-// One stage of every active solve: ghost fill of `in`, y faces, x faces fused
-// with the rhs and the update into `out` (Un = buffer 0).
-int stage(wenofd_batch* b, const BatchView& v, int in, int out, int mode, double a, double bc, double c, bool last) {
+int devloop_steps(DevLoop& L, const BatchView& v, int n, hipStream_t s) {
     int rc;
-    if ((rc = check_launch(launch_batch_ghost(v, in, b->tab_ghost, b->n_ghost, b->stream)))) return rc;
-    if ((rc = check_launch(launch_batch_flux_y(b->scheme, b->tiles, v, in, b->tab_y, b->n_y, b->stream)))) return rc;
-    return check_launch(launch_batch_flux_x_rk(b->scheme, b->tiles, v, in, out, mode, a, bc, c, last, b->tab_x,
-                                               b->n_x, b->stream));
-}
-
-// One iteration of the run() loop for every solve (src/WENOFD.cpp:213-239):
-// calculateDeltaT, RunRK<RKMETH>, now += dt. The RK coefficients are those of
-// src/WENOFD.cpp:301-396 (cdt = c * dt formed in the kernel).
-int enqueue_step(wenofd_batch* b) {
-    const BatchView v = b->view();
-    int rc;
-    if ((rc = check_launch(launch_batch_dt(v, b->stream)))) return rc;
-    if (b->rkmeth == 0) {
-        if ((rc = stage(b, v, 0, 1, kRk1, 0.0, 1.0, 1.0, true))) return rc;
-        if ((rc = check_launch(launch_batch_copy(b->tiles, v, 1, 0, b->tab_x, b->n_x, b->stream)))) return rc;
-    } else if (b->rkmeth == 1) {
-        if ((rc = stage(b, v, 0, 1, kRkStage, 1.0, 0.0, 1.0, false))) return rc;
-        if ((rc = stage(b, v, 1, 0, kRkStage, 0.5, 0.5, 0.5, true))) return rc;
-    } else {
-        if ((rc = stage(b, v, 0, 1, kRkStage, 1.0, 0.0, 1.0, false))) return rc;
-        if ((rc = stage(b, v, 1, 2, kRkStage, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, false))) return rc;
-        if ((rc = stage(b, v, 2, 0, kRkStage, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, true))) return rc;
-    }
-    return check_launch(launch_batch_advance(v, false, b->stream));
-}
-
-int ensure_graph(wenofd_batch* b) {
-    if (b->exec || b->graph_steps <= 0) return 0;
-    BHIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = 0;
-    for (int k = 0; k < b->graph_steps && !rc; ++k) rc = enqueue_step(b);
-    hipGraph_t graph = nullptr;
-    const hipError_t e = hipStreamEndCapture(b->stream, &graph);
-    if (rc) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return rc;
-    }
-    if (e != hipSuccess) return fail(WENOFD_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-    const hipError_t ei = hipGraphInstantiateWithFlags(&b->exec, graph, 0);
-    (void)hipGraphDestroy(graph);
-    if (ei != hipSuccess) {
-        b->exec = nullptr;
-        return fail(WENOFD_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
-    }
-    return 0;
-}
-
-// Enqueues n steps: whole graph replays, then single steps for the remainder.
-int enqueue_steps(wenofd_batch* b, int n) {
-    int rc;
-    if (b->graph_steps > 0 && n >= b->graph_steps) {
-        if ((rc = ensure_graph(b))) return rc;
-        for (; n >= b->graph_steps; n -= b->graph_steps) BHIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    if (L.graph_steps > 0 && n >= L.graph_steps) {
+        if ((rc = ensure_graph(L, v, s))) return rc;
+        for (; n >= L.graph_steps; n -= L.graph_steps) BHIP_TRY(hipGraphLaunch(L.exec, s));
     }
     for (; n > 0; --n)
-        if ((rc = enqueue_step(b))) return rc;
+        if ((rc = enqueue_step(L, v, s))) return rc;
     return 0;
 }
+
+}  // namespace wenofd
+
+namespace {
+
+int build_tables(wenofd_batch* b) { return devloop_tables(b->loop, b->desc, b->desc_d, b->part_cap); }
+
+int prime(wenofd_batch* b) { return devloop_prime(b->view(), b->stream); }
+
+int enqueue_steps(wenofd_batch* b, int n) { return devloop_steps(b->loop, b->view(), n, b->stream); }
+
+int ensure_graph(wenofd_batch* b) { return ensure_graph(b->loop, b->view(), b->stream); }
 
 int read_active(wenofd_batch* b, int* active) {
     BHIP_TRY(hipMemcpyAsync(b->active_h, b->active_d, sizeof(int), hipMemcpyDeviceToHost, b->stream));
@@ -250,8 +262,8 @@ int create_impl(const char* const* texts, int n, int device, wenofd_batch** out)
         if (rc) return bail(fail(rc, "solve " + std::to_string(s) + ": " + err));
         const Options& o = b->opt[s];
         if (s == 0) {
-            b->scheme = o.scheme;
-            b->rkmeth = o.rkmeth;
+            b->scheme = b->loop.scheme = o.scheme;
+            b->rkmeth = b->loop.rkmeth = o.rkmeth;
         } else if (o.scheme != b->scheme || o.rkmeth != b->rkmeth) {
             return bail(fail(WENOFD_E_CONFIG, "all solves of a batch must share SCHEME and RKMETH"));
         }
@@ -321,7 +333,7 @@ int set_tiles_impl(wenofd_batch* b, int x, int y) {
     t.y = y;
     if (!tiles_valid(t)) return fail(WENOFD_E_ARG, "tile shape must be x in {64, 128, 256}, y in {16, 32, 64}");
     BHIP_TRY(hipStreamSynchronize(b->stream));
-    b->tiles = t;
+    b->loop.tiles = t;
     int rc = build_tables(b);
     return rc ? rc : prime(b);  // the partials are laid out per x tile
 }
@@ -340,16 +352,13 @@ int wenofd_batch_destroy(wenofd_batch* b) {
     if (!b) return 0;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    drop_graph(b);
+    devloop_free(b->loop);
     for (double* p : {b->U[0], b->U[1], b->U[2], b->U_snap, b->Fy, b->special, b->part}) (void)hipFree(p);
     (void)hipFree(b->part_nan);
     (void)hipFree(b->desc_d);
     (void)hipFree(b->st_d);
     (void)hipFree(b->st_snap);
     (void)hipFree(b->active_d);
-    (void)hipFree(b->tab_ghost);
-    (void)hipFree(b->tab_y);
-    (void)hipFree(b->tab_x);
     if (b->st_h) (void)hipHostFree(b->st_h);
     if (b->active_h) (void)hipHostFree(b->active_h);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -377,8 +386,8 @@ int wenofd_batch_set_tiles(wenofd_batch* b, int x_faces, int y_faces) {
 
 int wenofd_batch_get_tiles(wenofd_batch* b, int* x_faces, int* y_faces) {
     if (!b || !x_faces || !y_faces) return fail(WENOFD_E_ARG, "null argument");
-    *x_faces = b->tiles.x;
-    *y_faces = b->tiles.y;
+    *x_faces = b->loop.tiles.x;
+    *y_faces = b->loop.tiles.y;
     return 0;
 }
 
@@ -386,8 +395,8 @@ int wenofd_batch_set_graph_steps(wenofd_batch* b, int steps) {
     if (!b || steps < 0 || steps > 1024) return fail(WENOFD_E_ARG, "graph steps must be in [0, 1024]");
     BHIP_TRY(hipSetDevice(b->device));
     BHIP_TRY(hipStreamSynchronize(b->stream));
-    drop_graph(b);
-    b->graph_steps = steps;
+    devloop_drop_graph(b->loop);
+    b->loop.graph_steps = steps;
     return 0;
 }
 
@@ -398,7 +407,7 @@ int wenofd_batch_advance(wenofd_batch* b, int max_steps, int* steps_done, int* a
     if ((rc = read_active(b, &active))) return rc;
     const auto t0 = std::chrono::steady_clock::now();
     while (active > 0 && (max_steps <= 0 || done < max_steps)) {
-        int chunk = b->graph_steps > 0 ? b->graph_steps : 64;
+        int chunk = b->loop.graph_steps > 0 ? b->loop.graph_steps : 64;
         if (max_steps > 0) chunk = std::min(chunk, max_steps - done);
         if ((rc = enqueue_steps(b, chunk))) return rc;
         done += chunk;
@@ -435,7 +444,7 @@ int wenofd_batch_autotune(wenofd_batch* b, int steps, int* x_faces, int* y_faces
     BHIP_TRY(hipEventCreate(&e1));
     int rc = 0;
     float best = 1e30f;
-    Tiles best_t = b->tiles;
+    Tiles best_t = b->loop.tiles;
     for (int x : {64, 128, 256}) {
         for (int y : {16, 32, 64}) {
             if ((rc = set_tiles_impl(b, x, y))) break;
@@ -449,7 +458,7 @@ int wenofd_batch_autotune(wenofd_batch* b, int steps, int* x_faces, int* y_faces
             if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = fail(WENOFD_E_HIP, "event time"); break; }
             if (ms < best) {
                 best = ms;
-                best_t = b->tiles;
+                best_t = b->loop.tiles;
             }
             if ((rc = restore())) break;
         }

@@ -16,6 +16,7 @@
 #include <algorithm>
 
 #include "wenofd_amd.h"
+#include "wenofd_devloop.hpp"
 #include "wenofd_host.hpp"
 #include "wenofd_kernels.hpp"
 #include "wenofd_mlp.hpp"
@@ -98,6 +99,18 @@ struct wenofd_solver {
     double** prev_slot = nullptr;
     double prev_dt = 0;
     long long fallbacks = 0;
+    // Device-side run loop for wenofd_advance (time step folded and applied on
+    // the device, steps replayed from a HIP graph; see wenofd_devloop.hpp).
+    DevLoop loop;
+    bool loop_on = true;
+    bool loop_ready = false;
+    std::vector<SolveDesc> loop_desc;
+    SolveDesc* loop_desc_d = nullptr;
+    SolveState* loop_st_d = nullptr;
+    SolveState* loop_st_h = nullptr;  // pinned
+    int* loop_active_d = nullptr;
+    int* loop_active_h = nullptr;  // pinned
+    size_t part_cap = 0;
 
     size_t state_doubles() const { return 4 * (size_t)g.plane; }
     long faces_x() const { return (long)g.ny * (g.nx + 1); }
@@ -370,6 +383,7 @@ int create_impl(const std::string& text, int device, wenofd_solver** out) {
     if (e == hipSuccess) e = hipMalloc(&h->special, 4 * (size_t)ns * sizeof(double));
     // room for the partials of the narrowest x tile
     const size_t nparts = std::max<size_t>(kDtBlocks, (size_t)flux_x_rk_blocks(g, Tiles{64, 64}));
+    h->part_cap = nparts;
     if (e == hipSuccess) e = hipMalloc(&h->dt_part, nparts * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&h->dt_part_nan, nparts * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&h->dt_dev, 2 * sizeof(double));
@@ -418,6 +432,161 @@ int ensure_out_dir(wenofd_solver* h) {
                      "/";
     }
     if (mkdirs(h->out_dir)) return fail(WENOFD_E_IO, "cannot create " + h->out_dir);
+    return 0;
+}
+
+BatchView loop_view(wenofd_solver* h) {
+    BatchView v;
+    v.desc = h->loop_desc_d;
+    v.st = h->loop_st_d;
+    v.U[0] = h->U;
+    v.U[1] = h->Ua;
+    v.U[2] = h->Ub;
+    v.Fy = h->Fy;
+    v.special = h->special;
+    v.part = h->dt_part;
+    v.part_nan = h->dt_part_nan;
+    v.active_count = h->loop_active_d;
+    v.nsolves = 1;
+    return v;
+}
+
+int loop_setup(wenofd_solver* h) {
+    if (h->loop_ready && h->loop.tiles.x == h->tiles.x && h->loop.tiles.y == h->tiles.y) return 0;
+    if (!h->loop_desc_d) {
+        HIP_TRY(hipMalloc(&h->loop_desc_d, sizeof(SolveDesc)));
+        HIP_TRY(hipMalloc(&h->loop_st_d, sizeof(SolveState)));
+        HIP_TRY(hipMalloc(&h->loop_active_d, sizeof(int)));
+        HIP_TRY(hipHostMalloc(&h->loop_st_h, sizeof(SolveState), hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc(&h->loop_active_h, sizeof(int), hipHostMallocDefault));
+    }
+    SolveDesc d{};
+    d.g = h->g;
+    d.bc = h->bc;
+    d.gravity = h->opt.testcase == kRTI ? 1 : 0;
+    d.cfl = h->opt.cfl;
+    d.t_end = h->cs.t_end;
+    d.part_off = 0;
+    d.nparts = (int)flux_x_rk_blocks(h->g, h->tiles);  // the partials of the x sweep's last stage
+    h->loop_desc.assign(1, d);
+    HIP_TRY(hipMemcpy(h->loop_desc_d, &d, sizeof(SolveDesc), hipMemcpyHostToDevice));
+    devloop_drop_graph(h->loop);
+    h->loop.tiles = h->tiles;
+    h->loop_ready = true;
+    return 0;
+}
+
+// One iteration of the run() loop on the device (src/WENOFD.cpp:213-239):
+// k_batch_dt folds the last stage's partials into the step (clipped to t_end,
+// the loop test |now - t_end| > 1e-9), the stages run the plain kernels with
+// the step read from device memory (nothing runs once the solve is done), and
+// k_batch_advance moves the clock. Same kernels and coefficients as run_rk.
+int enqueue_device_step(wenofd_solver* h, const BatchView& v) {
+    const SolveState* st = h->loop_st_d;
+    const int grav = h->opt.testcase == kRTI ? 1 : 0;
+    auto chk = [](hipError_t e) {
+        return e == hipSuccess ? 0 : fail(WENOFD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    };
+    auto stage_dev = [&](double* in, const double* un, double* out, const RkCoef& k, bool last) -> int {
+        int rc;
+        if ((rc = chk(launch_ghost_fill(in, h->special, h->g, h->bc, h->stream, st)))) return rc;
+        if ((rc = chk(launch_flux_y(h->opt.scheme, h->tiles, in, h->Fy, h->g, h->stream, nullptr, st)))) return rc;
+        return chk(launch_flux_x_rk(h->opt.scheme, h->tiles, in, h->Fy, un, out, nullptr, h->g, k, h->opt.cfl,
+                                    last ? h->dt_part : nullptr, last ? h->dt_part_nan : nullptr, h->stream, nullptr,
+                                    st));
+    };
+    int rc;
+    if ((rc = chk(launch_batch_dt(v, h->stream)))) return rc;
+    // RkCoef.cdt carries the stage coefficient c; the kernels multiply by dt
+    if (h->opt.rkmeth == 0) {
+        if ((rc = stage_dev(h->U, nullptr, h->Ua, RkCoef{kRk1, grav, 0.0, 1.0, 1.0}, true))) return rc;
+        if ((rc = chk(launch_copy_interior(h->Ua, h->U, h->g, st, h->stream)))) return rc;
+    } else if (h->opt.rkmeth == 1) {
+        if ((rc = stage_dev(h->U, h->U, h->Ua, RkCoef{kRkStage, grav, 1.0, 0.0, 1.0}, false))) return rc;
+        if ((rc = stage_dev(h->Ua, h->U, h->U, RkCoef{kRkStage, grav, 0.5, 0.5, 0.5}, true))) return rc;
+    } else {
+        if ((rc = stage_dev(h->U, h->U, h->Ua, RkCoef{kRkStage, grav, 1.0, 0.0, 1.0}, false))) return rc;
+        if ((rc = stage_dev(h->Ua, h->U, h->Ub, RkCoef{kRkStage, grav, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0}, false)))
+            return rc;
+        if ((rc = stage_dev(h->Ub, h->U, h->U, RkCoef{kRkStage, grav, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}, true)))
+            return rc;
+    }
+    return chk(launch_batch_advance(v, false, h->stream));
+}
+
+int enqueue_device_steps(wenofd_solver* h, const BatchView& v, int n) {
+    DevLoop& L = h->loop;
+    int rc;
+    if (L.graph_steps > 0 && n >= L.graph_steps) {
+        if (L.exec && (L.graph_U[0] != h->U || L.graph_U[1] != h->Ua || L.graph_U[2] != h->Ub))
+            devloop_drop_graph(L);  // the graph holds the buffer addresses it was captured with
+        if (!L.exec) {
+            HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            rc = 0;
+            for (int k = 0; k < L.graph_steps && !rc; ++k) rc = enqueue_device_step(h, v);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(WENOFD_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            const hipError_t ei = hipGraphInstantiateWithFlags(&L.exec, graph, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) {
+                L.exec = nullptr;
+                return fail(WENOFD_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+            }
+            L.graph_U[0] = h->U;
+            L.graph_U[1] = h->Ua;
+            L.graph_U[2] = h->Ub;
+        }
+        for (; n >= L.graph_steps; n -= L.graph_steps) HIP_TRY(hipGraphLaunch(L.exec, h->stream));
+    }
+    for (; n > 0; --n)
+        if ((rc = enqueue_device_step(h, v))) return rc;
+    return 0;
+}
+
+// wenofd_advance on the device loop: the same steps as the host loop (the
+// batch kernels run the same tile functions, the time step is the same exact
+// minimum), without a host round trip per step.
+int advance_on_device(wenofd_solver* h, int max_steps, int* steps_done) {
+    int rc;
+    *steps_done = 0;
+    if ((rc = loop_setup(h))) return rc;
+    const BatchView v = loop_view(h);
+    *h->loop_st_h = SolveState{h->now, 0.0, h->steps, 0, 0};
+    HIP_TRY(hipMemcpyAsync(h->loop_st_d, h->loop_st_h, sizeof(SolveState), hipMemcpyHostToDevice, h->stream));
+    if (h->dt_parts > 0) {  // the last stage's partials describe the current state
+        HIP_TRY(launch_batch_advance(v, true, h->stream));
+    } else if ((rc = devloop_prime(v, h->stream))) {
+        return rc;
+    }
+    const long long steps0 = h->steps;
+    int done = 0, active = 0;
+    auto read_active = [&]() -> int {
+        HIP_TRY(hipMemcpyAsync(h->loop_active_h, h->loop_active_d, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        active = *h->loop_active_h;
+        return 0;
+    };
+    if ((rc = read_active())) return rc;
+    while (active > 0 && (max_steps <= 0 || done < max_steps)) {
+        int chunk = h->loop.graph_steps > 0 ? h->loop.graph_steps : 64;
+        if (max_steps > 0) chunk = std::min(chunk, max_steps - done);
+        if ((rc = enqueue_device_steps(h, v, chunk))) return rc;
+        done += chunk;
+        if ((rc = read_active())) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(h->loop_st_h, h->loop_st_d, sizeof(SolveState), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->now = h->loop_st_h->now;
+    h->steps = h->loop_st_h->steps;
+    h->dt_parts = h->loop_desc[0].nparts;
+    h->prev_slot = nullptr;
+    *steps_done = (int)(h->steps - steps0);
+    if (h->loop_st_h->nan) return fail(WENOFD_E_NAN_DT, "Timestep is not a number (non-physical state)");
     return 0;
 }
 
@@ -487,6 +656,12 @@ int wenofd_destroy(wenofd_solver* h) {
     (void)hipFree(h->dt_part);
     (void)hipFree(h->dt_part_nan);
     (void)hipFree(h->dt_dev);
+    devloop_drop_graph(h->loop);
+    (void)hipFree(h->loop_desc_d);
+    (void)hipFree(h->loop_st_d);
+    (void)hipFree(h->loop_active_d);
+    if (h->loop_st_h) (void)hipHostFree(h->loop_st_h);
+    if (h->loop_active_h) (void)hipHostFree(h->loop_active_h);
     (void)hipFree(h->sw_params);
     (void)hipFree(h->fl_raw_x);
     (void)hipFree(h->fl_raw_y);
@@ -613,6 +788,12 @@ int wenofd_step(wenofd_solver* h, double dt_fixed, double* dt_used) {
 int wenofd_advance(wenofd_solver* h, int max_steps, double dt_fixed, int* steps_done) {
     if (!h) return fail(WENOFD_E_ARG, "null handle");
     int n = 0, rc = 0;
+    if (h->loop_on && !(dt_fixed > 0) && h->sw_mode == WENOFD_SWITCH_OFF && !h->timing) {
+        if ((rc = set_dev(h))) return rc;
+        rc = advance_on_device(h, max_steps, &n);
+        if (steps_done) *steps_done = n;
+        return rc;
+    }
     while (std::fabs(h->now - h->cs.t_end) > 1e-9) {  // src/WENOFD.cpp:213
         if (max_steps > 0 && n >= max_steps) break;
         if ((rc = wenofd_step(h, dt_fixed, nullptr))) break;
@@ -779,6 +960,17 @@ int wenofd_get_timing(wenofd_solver* h, double* ms, long long* counts, int nkind
         h->t_ms[k] = 0;
         h->t_cnt[k] = 0;
     }
+    return 0;
+}
+
+int wenofd_set_device_loop(wenofd_solver* h, int graph_steps) {
+    if (!h || graph_steps < 0 || graph_steps > 1024) return fail(WENOFD_E_ARG, "graph steps must be in [0, 1024]");
+    int rc = set_dev(h);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    devloop_drop_graph(h->loop);
+    h->loop_on = graph_steps > 0;
+    h->loop.graph_steps = graph_steps;
     return 0;
 }
 

@@ -88,8 +88,12 @@ __device__ __forceinline__ void ghost_point(double* __restrict__ U, const double
     }
 }
 
+// st (optional): the solve's device-side time state; an inactive solve skips
+// the launch (device loop, see wenofd_devloop.hpp).
 __global__ __launch_bounds__(kGhostThreads) void k_ghost_fill(double* __restrict__ U,
-                                                              const double* __restrict__ special, Geom g, Bc bc) {
+                                                              const double* __restrict__ special, Geom g, Bc bc,
+                                                              const SolveState* __restrict__ st) {
+    if (st && !st->active) return;
     ghost_point(U, special, g, bc, blockIdx.x * kGhostThreads + threadIdx.x);
 }
 
@@ -214,7 +218,9 @@ __host__ __device__ inline int y_tiles_y(const Geom& g) { return (g.ny + 1 + YF 
 
 template <int S, int YF, bool HYB>
 __global__ __launch_bounds__(kYC* kYR) void k_flux_y(const double* __restrict__ U, double* __restrict__ Fy, Geom g,
-                                                     const unsigned char* __restrict__ flags) {
+                                                     const unsigned char* __restrict__ flags,
+                                                     const SolveState* __restrict__ st) {
+    if (st && !st->active) return;
     flux_y_tile<S, YF, HYB>(U, Fy, g, blockIdx.x, blockIdx.y, flags);
 }
 
@@ -374,11 +380,17 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
 template <int XT>
 __host__ __device__ inline int x_tiles_x(const Geom& g) { return (g.nx + (XT - 1) - 1) / (XT - 1); }
 
+// With st, k.cdt holds the stage coefficient c and the step is c * st->dt.
 template <int S, int XT, bool HYB>
 __global__ __launch_bounds__(XT) void k_flux_x_rk(const double* __restrict__ Uin, const double* __restrict__ Fy,
                                                   const double* Un, double* Uout, double* __restrict__ rhs_out,
                                                   Geom g, RkCoef k, double cfl, double* __restrict__ dt_part,
-                                                  int* __restrict__ dt_nan, const unsigned char* __restrict__ flags) {
+                                                  int* __restrict__ dt_nan, const unsigned char* __restrict__ flags,
+                                                  const SolveState* __restrict__ st) {
+    if (st) {
+        if (!st->active) return;
+        k.cdt = k.cdt * st->dt;
+    }
     const long b = (long)blockIdx.y * gridDim.x + blockIdx.x;
     flux_x_rk_tile<S, XT, HYB>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part ? dt_part + b : nullptr,
                                dt_part ? dt_nan + b : nullptr, blockIdx.x, blockIdx.y, flags);
@@ -517,26 +529,32 @@ __global__ __launch_bounds__(XT) void k_batch_flux_x_rk(BatchView b, int in, int
                           d.cfl, part, pnan, tr.tile % ntx, tr.tile / ntx);
 }
 
-// First step: each solve's CFL minimum straight from its state, into the first
-// of its partial slots (the rest are set neutral).
-__global__ __launch_bounds__(kDtFinalThreads) void k_batch_dt_partial(BatchView b) {
-    __shared__ double sm[kDtFinalThreads / 64];
-    __shared__ int sn[kDtFinalThreads / 64];
-    const SolveDesc& d = b.desc[blockIdx.x];
+// First step: each solve's CFL minima straight from its state. Block
+// (bx, solve) of gridDim.x reduces the cells bx, bx + gridDim.x, ... (in units
+// of 256) into partial slot bx when bx < nparts, and sets the slots
+// >= gridDim.x that fall to it (q = bx mod gridDim.x) neutral.
+constexpr int kPrimeBlocks = 256;
+
+__global__ __launch_bounds__(256) void k_batch_dt_partial(BatchView b) {
+    __shared__ double sm[4];
+    __shared__ int sn[4];
+    const SolveDesc& d = b.desc[blockIdx.y];
+    const int nb = min((int)gridDim.x, d.nparts);
+    if ((int)blockIdx.x >= nb) return;
     const double* U = b.U[0] + d.u_off;
     double m = 1.0;
     int nan = 0;
     const long total = (long)d.g.nx * d.g.ny;
-    for (long k = threadIdx.x; k < total; k += kDtFinalThreads) dt_point(U, d.g, d.cfl, k, m, nan);
-    min_nan_reduce_block<kDtFinalThreads / 64>(m, nan, sm, sn);
-    for (int q = threadIdx.x; q < d.nparts; q += kDtFinalThreads) {
+    for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < total; k += (long)nb * 256)
+        dt_point(U, d.g, d.cfl, k, m, nan);
+    min_nan_reduce_block<4>(m, nan, sm, sn);
+    for (int q = nb + blockIdx.x + threadIdx.x * nb; q < d.nparts; q += nb * 256) {
         b.part[d.part_off + q] = 1.0;
         b.part_nan[d.part_off + q] = 0;
     }
-    __syncthreads();
     if (threadIdx.x == 0) {
-        b.part[d.part_off] = m;
-        b.part_nan[d.part_off] = nan;
+        b.part[d.part_off + blockIdx.x] = m;
+        b.part_nan[d.part_off + blockIdx.x] = nan;
     }
 }
 
@@ -603,6 +621,17 @@ __global__ __launch_bounds__(XT) void k_batch_copy(BatchView b, int from, int to
     const size_t o = d.u_off + (size_t)j * d.g.pitch + i;
 #pragma unroll
     for (int r = 0; r < 4; ++r) b.U[to][o + r * d.g.plane] = b.U[from][o + r * d.g.plane];
+}
+
+// Interior copy from -> to of one solve (forward Euler on the device loop).
+__global__ __launch_bounds__(256) void k_copy_interior(const double* __restrict__ from, double* __restrict__ to, Geom g,
+                                                       const SolveState* __restrict__ st) {
+    if (st && !st->active) return;
+    const int i = 4 + blockIdx.x * 256 + threadIdx.x, j = 4 + blockIdx.y;
+    if (i >= g.nx + 4) return;
+    const size_t o = (size_t)j * g.pitch + i;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) to[r * g.plane + o] = from[r * g.plane + o];
 }
 
 // ---------------------------------------------------------------------------
@@ -686,36 +715,43 @@ long flux_x_rk_blocks(const Geom& g, const Tiles& t) {
     return n;
 }
 
-hipError_t launch_ghost_fill(double* U, const double* special, const Geom& g, const Bc& bc, hipStream_t s) {
-    hipLaunchKernelGGL(k_ghost_fill, dim3(ghost_blocks(g)), dim3(kGhostThreads), 0, s, U, special, g, bc);
+hipError_t launch_ghost_fill(double* U, const double* special, const Geom& g, const Bc& bc, hipStream_t s,
+                             const SolveState* st) {
+    hipLaunchKernelGGL(k_ghost_fill, dim3(ghost_blocks(g)), dim3(kGhostThreads), 0, s, U, special, g, bc, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_interior(const double* from, double* to, const Geom& g, const SolveState* st, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_interior, dim3((g.nx + 255) / 256, g.ny), dim3(256), 0, s, from, to, g, st);
     return hipGetLastError();
 }
 
 hipError_t launch_flux_y(int scheme, const Tiles& t, const double* U, double* Fy, const Geom& g, hipStream_t s,
-                         const unsigned char* flags) {
+                         const unsigned char* flags, const SolveState* st) {
     return with_scheme(scheme, [&](auto S) {
         with_y_tile(t.y, [&](auto YF) {
             dim3 grid(y_tiles_x<YF()>(g), y_tiles_y<YF()>(g));
             if (flags)
-                hipLaunchKernelGGL((k_flux_y<S(), YF(), true>), grid, dim3(kYC * kYR), 0, s, U, Fy, g, flags);
+                hipLaunchKernelGGL((k_flux_y<S(), YF(), true>), grid, dim3(kYC * kYR), 0, s, U, Fy, g, flags, st);
             else
-                hipLaunchKernelGGL((k_flux_y<S(), YF(), false>), grid, dim3(kYC * kYR), 0, s, U, Fy, g, flags);
+                hipLaunchKernelGGL((k_flux_y<S(), YF(), false>), grid, dim3(kYC * kYR), 0, s, U, Fy, g, flags, st);
         });
     });
 }
 
 hipError_t launch_flux_x_rk(int scheme, const Tiles& t, const double* Uin, const double* Fy, const double* Un,
                             double* Uout, double* rhs_out, const Geom& g, const RkCoef& k, double cfl,
-                            double* dt_part, int* dt_nan, hipStream_t s, const unsigned char* flags) {
+                            double* dt_part, int* dt_nan, hipStream_t s, const unsigned char* flags,
+                            const SolveState* st) {
     return with_scheme(scheme, [&](auto S) {
         with_x_tile(t.x, [&](auto XT) {
             dim3 grid(x_tiles_x<XT()>(g), g.ny);
             if (flags)
                 hipLaunchKernelGGL((k_flux_x_rk<S(), XT(), true>), grid, dim3(XT()), 0, s, Uin, Fy, Un, Uout,
-                                   rhs_out, g, k, cfl, dt_part, dt_nan, flags);
+                                   rhs_out, g, k, cfl, dt_part, dt_nan, flags, st);
             else
                 hipLaunchKernelGGL((k_flux_x_rk<S(), XT(), false>), grid, dim3(XT()), 0, s, Uin, Fy, Un, Uout,
-                                   rhs_out, g, k, cfl, dt_part, dt_nan, flags);
+                                   rhs_out, g, k, cfl, dt_part, dt_nan, flags, st);
         });
     });
 }
@@ -769,7 +805,7 @@ hipError_t launch_batch_flux_x_rk(int scheme, const Tiles& t, const BatchView& b
 }
 
 hipError_t launch_batch_dt_partial(const BatchView& b, hipStream_t s) {
-    hipLaunchKernelGGL(k_batch_dt_partial, dim3(b.nsolves), dim3(kDtFinalThreads), 0, s, b);
+    hipLaunchKernelGGL(k_batch_dt_partial, dim3(kPrimeBlocks, b.nsolves), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 

@@ -40,17 +40,25 @@ bool tiles_valid(const Tiles& t);
 
 constexpr int kProbePhysStride = 82;
 
+struct SolveState;
+
 // ---- one solve --------------------------------------------------------------
-hipError_t launch_ghost_fill(double* U, const double* special, const Geom& g, const Bc& bc, hipStream_t s);
+// st (optional, device): the solve's time state on the device loop; the
+// launches do nothing for an inactive solve and the x sweep takes its step
+// as k.cdt * st->dt.
+hipError_t launch_ghost_fill(double* U, const double* special, const Geom& g, const Bc& bc, hipStream_t s,
+                             const SolveState* st = nullptr);
+hipError_t launch_copy_interior(const double* from, double* to, const Geom& g, const SolveState* st, hipStream_t s);
 // flags (optional): per-face switch flags of the direction (x faces
 // [ny][nx+1], y faces [ny+1][nx]); zero selects the upwind-5 flux.
 hipError_t launch_flux_y(int scheme, const Tiles& t, const double* U, double* Fy, const Geom& g, hipStream_t s,
-                         const unsigned char* flags = nullptr);
+                         const unsigned char* flags = nullptr, const SolveState* st = nullptr);
 // x faces + rhs + RK stage. dt_part (optional) receives one partial CFL
 // minimum per block: flux_x_rk_blocks(g, t) entries.
 hipError_t launch_flux_x_rk(int scheme, const Tiles& t, const double* Uin, const double* Fy, const double* Un,
                             double* Uout, double* rhs_out, const Geom& g, const RkCoef& k, double cfl,
-                            double* dt_part, int* dt_nan, hipStream_t s, const unsigned char* flags = nullptr);
+                            double* dt_part, int* dt_nan, hipStream_t s, const unsigned char* flags = nullptr,
+                            const SolveState* st = nullptr);
 long flux_x_rk_blocks(const Geom& g, const Tiles& t);
 long flux_y_blocks(const Geom& g, const Tiles& t);
 long ghost_blocks(const Geom& g);

@@ -29,7 +29,7 @@ EXPORTS = (
     "wenofd_get_state", "wenofd_set_state", "wenofd_get_info", "wenofd_set_time", "wenofd_output_dir",
     "wenofd_output_ave", "wenofd_output_error", "wenofd_output_accuracy", "wenofd_accuracy",
     "wenofd_set_timing", "wenofd_get_timing", "wenofd_synchronize", "wenofd_probe_weno5",
-    "wenofd_probe_physics", "wenofd_set_tiles",
+    "wenofd_probe_physics", "wenofd_set_tiles", "wenofd_set_device_loop",
     "wenofd_batch_create", "wenofd_batch_destroy", "wenofd_batch_size", "wenofd_batch_initialize_ave",
     "wenofd_batch_advance", "wenofd_batch_set_graph_steps", "wenofd_batch_set_tiles", "wenofd_batch_get_tiles",
     "wenofd_batch_autotune", "wenofd_batch_get_state", "wenofd_batch_get_info", "wenofd_batch_accuracy",
@@ -94,6 +94,7 @@ def lib():
         L.wenofd_probe_weno5.argtypes = [i, i, d, i, _dp, _dp]
         L.wenofd_probe_physics.argtypes = [i, d, i, _dp, _dp]
         L.wenofd_set_tiles.argtypes = [vp, i, i]
+        L.wenofd_set_device_loop.argtypes = [vp, i]
         ip = ctypes.POINTER(i)
         L.wenofd_batch_create.argtypes = [ctypes.POINTER(ctypes.c_char_p), i, i, ctypes.POINTER(vp)]
         for n in ("wenofd_batch_destroy", "wenofd_batch_initialize_ave", "wenofd_batch_synchronize"):
@@ -313,6 +314,10 @@ class Solver:
 
     def set_tiles(self, x_faces, y_faces):
         _check(lib().wenofd_set_tiles(self.h, int(x_faces), int(y_faces)))
+
+    def set_device_loop(self, graph_steps):
+        """Steps per graph replay of advance()'s device loop; 0: one host-driven step at a time."""
+        _check(lib().wenofd_set_device_loop(self.h, int(graph_steps)))
 
     def synchronize(self):
         _check(lib().wenofd_synchronize(self.h))

@@ -116,8 +116,14 @@ int wenofd_get_rhs(wenofd_solver* h, double* host, int layout);
  * calculateDeltaT (or dt_fixed if > 0), RunRK<RKMETH>(dt), now += dt. */
 int wenofd_step(wenofd_solver* h, double dt_fixed, double* dt_used);
 /* Repeats wenofd_step while |now - t_end| > 1e-9 and fewer than max_steps
- * (max_steps <= 0: no bound) steps were taken. */
+ * (max_steps <= 0: no bound) steps were taken. With CFL time steps, the
+ * switch off and timing off, the steps run on the device loop: each time step
+ * is folded and applied on the device and the steps are replayed from a HIP
+ * graph, with no host round trip per step (same results bit for bit). */
 int wenofd_advance(wenofd_solver* h, int max_steps, double dt_fixed, int* steps_done);
+/* Steps per graph replay of wenofd_advance's device loop (default 16); 0 runs
+ * every step from the host (one dt read-back per step, as the reference's loop). */
+int wenofd_set_device_loop(wenofd_solver* h, int graph_steps);
 /* CWENOFD::run (src/WENOFD.cpp:189-299) after creation: outputAve("initial"),
  * the loop to t_end with progress lines every 20 steps (verbose != 0), then
  * outputAve("final") and, when an exact solution exists, outputAccuracy and

@@ -1,9 +1,11 @@
 """Batched solves (wenofd_batch_*) against the single-solve path and the oracle.
 
 Bar: every solve of a batch equals the same config run alone through
-wenofd_advance bit for bit (IEEE ==), whatever the tile shape, the graph
-length and the other members of the batch; the single-solve path itself is
-pinned bit for bit to the oracle and the reference in test_gpu_parity.py.
+wenofd_advance's host-driven loop bit for bit (IEEE ==), whatever the tile
+shape, the graph length and the other members of the batch; the single-solve
+path itself is pinned bit for bit to the oracle and the reference in
+test_gpu_parity.py. The single solver's device loop (the batch machinery over
+one solve) equals its host loop bit for bit.
 """
 import numpy as np
 import pytest
@@ -22,9 +24,11 @@ def cfg(tc, sc, rk, cfl, nx, ny, **extra):
 
 
 def single_states(configs, steps):
+    """The single-solve path, one host-driven step at a time (the plain kernels)."""
     out = []
     for c in configs:
         with W.Solver(c) as s:
+            s.set_device_loop(0)
             s.advance(steps)
             out.append((s.get_state(), s.info()))
     return out
@@ -66,6 +70,7 @@ def test_tile_shapes_are_bitwise_identical(tx, ty):
         check_batch(b, ref, f"tiles {tx}x{ty}")
     with W.Solver(cfgs[0]) as s:  # the single-solve path under the same shape
         s.set_tiles(tx, ty)
+        s.set_device_loop(0)
         s.advance(9)
         assert same(s.get_state(), ref[0][0])
 
@@ -119,6 +124,7 @@ def test_solves_stop_at_their_own_end_time():
     ref = []
     for c in cfgs:
         with W.Solver(c) as s:
+            s.set_device_loop(0)
             s.advance(0)
             ref.append((s.get_state(), s.info()))
     with W.Batch(cfgs) as b:
@@ -169,3 +175,37 @@ def test_nan_step_stops_only_that_solve():
         assert b.info(0).steps == 3 and b.info(1).steps == 0
         ref = single_states([good], 3)
         assert same(b.get_state(0), ref[0][0])
+
+
+@pytest.mark.parametrize("tc,rk,steps", [(4, 2, 37), (5, 0, 21), (5, 1, 18), (3, 2, 16), (0, 2, 40)])
+def test_single_solver_device_loop_equals_host_loop(tc, rk, steps):
+    """wenofd_advance on the device loop (graph replays, dt on the device) gives
+    the host-driven loop's state, time and step count bit for bit, including a
+    stop at t_end inside a graph replay and a later call that continues."""
+    extra = {"QUAD_SEED": 4} if tc == 5 else {}
+    c = cfg(tc, 1, rk, 0.5, 26, 22, **extra)
+    print(f"[gpu] device loop tc={tc} rk={rk}", flush=True)
+    with W.Solver(c) as a, W.Solver(c) as b:
+        a.set_device_loop(0)
+        assert a.advance(steps) == b.advance(steps) == steps
+        assert same(a.get_state(), b.get_state()) and a.info().now == b.info().now
+        a.step()  # a host step in between: the device loop picks up its partials
+        b.step()
+        assert a.advance(5) == b.advance(5) == 5
+        assert same(a.get_state(), b.get_state())
+        assert a.info().steps == b.info().steps == steps + 6
+    t_end = 0.0375
+    c2 = cfg(tc, 1, rk, 0.5, 26, 22, TEND="%.17g" % t_end, **extra)
+    with W.Solver(c2) as a, W.Solver(c2) as b:
+        a.set_device_loop(0)
+        na, nb = a.advance(0), b.advance(0)
+        assert na == nb and same(a.get_state(), b.get_state())
+        assert abs(b.info().now - t_end) <= 1e-9 and b.advance(3) == 0
+
+
+def test_device_loop_reports_a_nan_step():
+    bad = cfg(5, 1, 2, 0.5, 16, 16, QUAD_STATES="1 0 0 1  1 0 0 1  1 0 0 -1  1 0 0 1")
+    with W.Solver(bad) as s:
+        with pytest.raises(W.WenofdError) as ei:
+            s.advance(5)
+        assert ei.value.code == W.E_NAN_DT and s.info().steps == 0
