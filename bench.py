@@ -220,6 +220,9 @@ def main():
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--scheme", default="Z", choices=list(SCHEMES))
     ap.add_argument("--cfl", type=float, default=0.6)
+    ap.add_argument("--case", default="vortex", choices=["vortex", "quadrant"],
+                    help="isentropic vortex (the headline) or a seeded four-quadrant problem")
+    ap.add_argument("--quad-seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for cpu_baseline (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=200, help="steps of the 64-solve batch (0: skip)")
     ap.add_argument("--seq-steps", type=int, default=20, help="steps per solve of the sequential comparison")
@@ -235,8 +238,12 @@ def main():
     import wenofd as W
 
     n, sc = args.size, SCHEMES[args.scheme]
-    log(f"vortex {n}x{n} WENO-{args.scheme} RK3 CFL {args.cfl}: create")
-    s = W.Solver(TESTCASE=4, SCHEME=sc, RKMETH=2, CFL=args.cfl, ELEMNUMX=n, ELEMNUMY=n, TEND=1e9)
+    log(f"{args.case} {n}x{n} WENO-{args.scheme} RK3 CFL {args.cfl}: create")
+    if args.case == "vortex":
+        s = W.Solver(TESTCASE=4, SCHEME=sc, RKMETH=2, CFL=args.cfl, ELEMNUMX=n, ELEMNUMY=n, TEND=1e9)
+    else:
+        s = W.Solver(TESTCASE=5, SCHEME=sc, RKMETH=2, CFL=args.cfl, ELEMNUMX=n, ELEMNUMY=n, TEND=1e9,
+                     QUAD_SEED=args.quad_seed)
     log(f"warmup {args.warmup} steps")
     s.advance(args.warmup)
     s.synchronize()
@@ -282,10 +289,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (isentropic vortex initial condition, generated on the host)",
-        "config": {"workload": f"isentropic vortex {n}x{n}, WENO-{args.scheme}, RK3, CFL {args.cfl}, "
-                               "dt from calculateDeltaT every step",
-                   "nx": n, "ny": n, "scheme": args.scheme, "rk": "RK3", "cfl": args.cfl},
+        "data": "synthetic (initial condition generated on the host)",
+        "config": {"workload": (f"isentropic vortex {n}x{n}" if args.case == "vortex" else
+                                f"seeded four-quadrant problem (seed {args.quad_seed}) {n}x{n}") +
+                               f", WENO-{args.scheme}, RK3, CFL {args.cfl}, dt from calculateDeltaT every step",
+                   "nx": n, "ny": n, "scheme": args.scheme, "rk": "RK3", "cfl": args.cfl, "case": args.case},
         "roofline": {
             "kernel": dom,
             "bound": "fp64",
@@ -293,9 +301,9 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": dk["tflops"] / FP64_PEAK_TFLOPS,
-            "traffic": measured_traffic(dom, n) if args.scheme == "Z" else None,
+            "traffic": measured_traffic(dom, n) if (args.scheme == "Z" and args.case == "vortex") else None,
             "hbm": {"achieved": dk["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dk["gbs"] / HBM_PEAK_GBS},
-            "valu_issue": valu_issue(dom, n, dk["avg_us"]) if args.scheme == "Z" else None,
+            "valu_issue": valu_issue(dom, n, dk["avg_us"]) if (args.scheme == "Z" and args.case == "vortex") else None,
             "step_fp64_tflops": step_flops / (el / args.steps) / 1e12,
             "kernels": kernels,
         },

@@ -149,7 +149,15 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
                                                    double zp_lambda, bool& ok) {
     double b0, b1, b2, w0, w1, w2, p0, p1, p2;
     weno_beta(a, b, c, d, e, b0, b1, b2);
-    if constexpr (S == kZ) {
+    if constexpr (S == kJS) {
+        // d_k / (eps + beta_k)^2: denominators >= 1e-12
+        const double eps = 1.0e-6;
+        const double d0 = (eps + b0) * (eps + b0), d1 = (eps + b1) * (eps + b1), d2 = (eps + b2) * (eps + b2);
+        ok = ok && d0 <= kDivSafeMax && d1 <= kDivSafeMax && d2 <= kDivSafeMax;
+        w0 = div_refined(0.1, d0, rcp_refined(d0));
+        w1 = div_refined(0.6, d1, rcp_refined(d1));
+        w2 = div_refined(0.3, d2, rcp_refined(d2));
+    } else if constexpr (S == kZ) {
         const double eps = 1.0e-20;
         const double d0 = b0 + eps, d1 = b1 + eps, d2 = b2 + eps;
         ok = ok && d0 <= kDivSafeMax && d1 <= kDivSafeMax && d2 <= kDivSafeMax;
@@ -161,6 +169,8 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
         w1 = 0.6 * (1.0 + (((tau * i1) * tau) * i1));
         w2 = 0.3 * (1.0 + (((tau * i2) * tau) * i2));
     } else {
+        // ZP / ZPI keep the IEEE divisions: their shared-reciprocal form costs
+        // more registers than it saves instructions (3 waves per SIMD lost)
         weno_alpha<S>(b0, b1, b2, zp_lambda, w0, w1, w2);
     }
     const double s = (w0 + w1) + w2;
