@@ -158,9 +158,10 @@ int devloop_tables(DevLoop& L, std::vector<SolveDesc>& desc, SolveDesc* desc_d, 
     for (int s = 0; s < (int)desc.size(); ++s) {
         SolveDesc& d = desc[s];
         const long ng = ghost_blocks(d.g), ny = flux_y_blocks(d.g, L.tiles), nx = flux_x_rk_blocks(d.g, L.tiles);
-        for (long k = 0; k < ng; ++k) tg.push_back({s, (int)k});
-        for (long k = 0; k < ny; ++k) ty.push_back({s, (int)k});
-        for (long k = 0; k < nx; ++k) tx.push_back({s, (int)k});
+        const int ntx = x_tiles_per_row(d.g, L.tiles), nty = y_tiles_per_row(d.g, L.tiles);
+        for (long k = 0; k < ng; ++k) tg.push_back({s, (int)k, 0, 0});
+        for (long k = 0; k < ny; ++k) ty.push_back({s, (int)k, (int)(k % nty), (int)(k / nty)});
+        for (long k = 0; k < nx; ++k) tx.push_back({s, (int)k, (int)(k % ntx), (int)(k / ntx)});
         d.part_off = part_off;
         d.nparts = (int)nx;
         part_off += nx;

@@ -504,36 +504,47 @@ __global__ __launch_bounds__(kDtFinalThreads) void k_dt_final(const double* __re
 // Batched phases. Block b works on tile tiles[b].tile of solve tiles[b].solve;
 // solves that are not active in this step return at once.
 // ---------------------------------------------------------------------------
+// The block's table entry, then its solve's descriptor and time state by
+// value, issued together behind the one entry load. False for a solve that is
+// no longer active.
+__device__ __forceinline__ bool batch_block(const BatchView& b, const TileRef* __restrict__ tiles, TileRef& tr,
+                                            SolveDesc& d, SolveState& st) {
+    tr = tiles[blockIdx.x];
+    d = b.desc[tr.solve];
+    st = b.st[tr.solve];
+    return st.active;
+}
+
 __global__ __launch_bounds__(kGhostThreads) void k_batch_ghost(BatchView b, int buf, const TileRef* __restrict__ tiles) {
-    const TileRef tr = tiles[blockIdx.x];
-    if (!b.st[tr.solve].active) return;
-    const SolveDesc& d = b.desc[tr.solve];
+    TileRef tr;
+    SolveDesc d;
+    SolveState st;
+    if (!batch_block(b, tiles, tr, d, st)) return;
     ghost_point(b.U[buf] + d.u_off, b.special + d.sp_off, d.g, d.bc, tr.tile * kGhostThreads + threadIdx.x);
 }
 
 template <int S, int YF>
 __global__ __launch_bounds__(kYC* kYR) void k_batch_flux_y(BatchView b, int buf, const TileRef* __restrict__ tiles) {
-    const TileRef tr = tiles[blockIdx.x];
-    if (!b.st[tr.solve].active) return;
-    const SolveDesc& d = b.desc[tr.solve];
-    const int ntx = y_tiles_x<YF>(d.g);
-    flux_y_tile<S, YF>(b.U[buf] + d.u_off, b.Fy + d.fy_off, d.g, tr.tile % ntx, tr.tile / ntx);
+    TileRef tr;
+    SolveDesc d;
+    SolveState st;
+    if (!batch_block(b, tiles, tr, d, st)) return;
+    flux_y_tile<S, YF>(b.U[buf] + d.u_off, b.Fy + d.fy_off, d.g, tr.bx, tr.by);
 }
 
 template <int S, int XT>
 __global__ __launch_bounds__(XT) void k_batch_flux_x_rk(BatchView b, int in, int out, int mode, double a,
                                                         double bcoef, double c_stage, int last,
                                                         const TileRef* __restrict__ tiles) {
-    const TileRef tr = tiles[blockIdx.x];
-    const SolveState& st = b.st[tr.solve];
-    if (!st.active) return;
-    const SolveDesc& d = b.desc[tr.solve];
+    TileRef tr;
+    SolveDesc d;
+    SolveState st;
+    if (!batch_block(b, tiles, tr, d, st)) return;
     RkCoef k{mode, d.gravity, a, bcoef, c_stage * st.dt};
-    const int ntx = x_tiles_x<XT>(d.g);
     double* part = last ? b.part + d.part_off + tr.tile : nullptr;
     int* pnan = last ? b.part_nan + d.part_off + tr.tile : nullptr;
     flux_x_rk_tile<S, XT>(b.U[in] + d.u_off, b.Fy + d.fy_off, b.U[0] + d.u_off, b.U[out] + d.u_off, nullptr, d.g, k,
-                          d.cfl, part, pnan, tr.tile % ntx, tr.tile / ntx);
+                          d.cfl, part, pnan, tr.bx, tr.by);
 }
 
 // First step: each solve's CFL minima straight from its state. Block
@@ -618,12 +629,12 @@ __global__ __launch_bounds__(256) void k_batch_advance(BatchView b, int count_on
 
 template <int XT>
 __global__ __launch_bounds__(XT) void k_batch_copy(BatchView b, int from, int to, const TileRef* __restrict__ tiles) {
-    const TileRef tr = tiles[blockIdx.x];
-    if (!b.st[tr.solve].active) return;
-    const SolveDesc& d = b.desc[tr.solve];
-    const int ntx = x_tiles_x<XT>(d.g);
-    const int i = 4 + (tr.tile % ntx) * (XT - 1) + threadIdx.x;
-    const int j = 4 + tr.tile / ntx;
+    TileRef tr;
+    SolveDesc d;
+    SolveState st;
+    if (!batch_block(b, tiles, tr, d, st)) return;
+    const int i = 4 + tr.bx * (XT - 1) + threadIdx.x;
+    const int j = 4 + tr.by;
     if (threadIdx.x >= XT - 1 || i >= d.g.nx + 4) return;
     const size_t o = d.u_off + (size_t)j * d.g.pitch + i;
 #pragma unroll
@@ -713,6 +724,18 @@ long ghost_blocks(const Geom& g) { return (8L * (g.nx + g.ny) + kGhostThreads - 
 long flux_y_blocks(const Geom& g, const Tiles& t) {
     long n = 0;
     with_y_tile(t.y, [&](auto YF) { n = (long)y_tiles_x<YF()>(g) * y_tiles_y<YF()>(g); });
+    return n;
+}
+
+int x_tiles_per_row(const Geom& g, const Tiles& t) {
+    int n = 0;
+    with_x_tile(t.x, [&](auto XT) { n = x_tiles_x<XT()>(g); });
+    return n;
+}
+
+int y_tiles_per_row(const Geom& g, const Tiles& t) {
+    int n = 0;
+    with_y_tile(t.y, [&](auto YF) { n = y_tiles_x<YF()>(g); });
     return n;
 }
 

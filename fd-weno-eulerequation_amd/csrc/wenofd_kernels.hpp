@@ -90,10 +90,15 @@ struct SolveState {
     int nan;     // a NaN time step was met (the solve stops)
 };
 
-// Block -> (solve, tile index within the solve) of one launch.
+// Block -> solve and tile of one launch: tile = the tile's index within the
+// solve (its dt partial slot in the x sweep), (bx, by) = its coordinates
+// (x sweep: tile in the row, row; y sweep: column tile, face-row tile).
 struct TileRef {
-    int solve, tile;
+    int solve, tile, bx, by;
 };
+// Tiles per row of the two sweeps (flux_x_rk_blocks = x_tiles_per_row * ny).
+int x_tiles_per_row(const Geom& g, const Tiles& t);
+int y_tiles_per_row(const Geom& g, const Tiles& t);
 
 struct BatchView {
     const SolveDesc* desc;
