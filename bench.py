@@ -4,8 +4,15 @@ Workload (one step = one iteration of the reference's run() loop,
 src/WENOFD.cpp:213-239): calculateDeltaT, then RunRK3 = 3 x (ghost fill, x-face
 fluxes, y-face fluxes, fused rhs + stage update), on the isentropic vortex on
 [0,10]^2 (periodic), 2048 x 2048 points, WENO-Z, CFL 0.6. The state is resident
-in HBM before the timed region; the only host traffic per step is the 8-byte
-time-step read-back the reference's loop needs.
+in HBM before the timed region; the time step is folded and applied on the
+device (the reference loop's calculateDeltaT + now += dt), steps are replayed
+from a HIP graph.
+
+FP64 arithmetic mode (--fp): "fused" (the default, config key FP_CONTRACT = 1)
+runs the sweeps built with FMA contraction, like the reference's own
+optimised CPU build; "exact" the build without contraction, bit-identical to
+the reference compiled without it. Both meet the drop-in bar (relative
+max-norm <= 1e-11 against the reference), and the line reports both rates.
 
 Prints one JSON line (see the fields below). --gpus > 1 is refused: this
 solver runs one whole domain on one GPU.
@@ -61,40 +68,45 @@ def kernel_algorithmic(kind, counts, scheme, nx, ny, rk_first=False):
     return flops, nbytes
 
 
-def measured_traffic(kernel, n):
+FP_MODES = {"exact": 0, "fused": 1}
+
+
+def _profile_line(path, tag):
+    path = os.path.join(ROOT, "profiles", path)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(tag + ":"):
+            return dict(kv.split("=") for kv in line.split(":", 1)[1].split())
+    return None
+
+
+def measured_traffic(kernel, n, fp):
     """HBM bytes per launch from the committed rocprofv3 counter summary
     (FETCH_SIZE + WRITE_SIZE, calibrated), or None when not measured for n."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_vortex2048_wenoz_rk3.md")
-    if n != 2048 or not os.path.exists(path):
+    if n != 2048:
         return None
-    for line in open(path):
-        if line.startswith("traffic_2048:"):
-            vals = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
-            return int(vals[kernel]) if kernel in vals else None
-    return None
+    vals = _profile_line("hbm_traffic_vortex2048_wenoz_rk3.md", "traffic_2048" + ("_fused" if fp == "fused" else ""))
+    return int(vals[kernel]) if vals and kernel in vals else None
 
 
-def measured_valu(kernel, n):
+def measured_valu(kernel, n, fp):
     """VALU wave-instructions per launch from the committed counter summary
     (SQ_INSTS_VALU), or None when not measured for n."""
-    path = os.path.join(ROOT, "profiles", "valu_vortex2048_wenoz_rk3.md")
-    if n != 2048 or not os.path.exists(path):
+    if n != 2048:
         return None
-    for line in open(path):
-        if line.startswith("valu_2048:"):
-            vals = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
-            return int(vals[kernel]) if kernel in vals else None
-    return None
+    vals = _profile_line("valu_vortex2048_wenoz_rk3.md", "valu_2048" + ("_fused" if fp == "fused" else ""))
+    return int(vals[kernel]) if vals and kernel in vals else None
 
 
 # FP64 VALU issue capacity: 256 CUs x 4 SIMDs, one wave64 FP64 instruction per 4 cycles, 2.4 GHz.
 VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
 
 
-def valu_issue(kernel, n, avg_us):
+def valu_issue(kernel, n, avg_us, fp):
     """Measured VALU wave-instructions per launch over this run's duration,
     against the chip's FP64 issue capacity."""
-    w = measured_valu(kernel, n)
+    w = measured_valu(kernel, n, fp)
     if w is None:
         return None
     achieved = w / (avg_us * 1e-6)
@@ -118,12 +130,12 @@ def cpu_baseline(nx, ny, scheme, cfl, steps):
                       f"{[k for k, v in SCHEMES.items() if v == scheme][0]}, serial C++ oracle, {el:.1f} s"}
 
 
-def batched(W, steps, seq_steps, seed=0):
+def batched(W, steps, seq_steps, fp, seed=0):
     """64 seeded vortex solves (N in {100, 120, ..., 400}, strength ~ U[3, 6],
     centre ~ U[3, 7]^2), WENO-Z RK3 CFL 0.05, advanced as one batch (one
     launch per phase, graph-captured steps, autotuned tiles) against the same
     64 solves run one after another through the single-solve path."""
-    cfgs = W.vortex_batch_configs(n=64, seed=seed, cfl=0.05)
+    cfgs = [c + f"FP_CONTRACT = {FP_MODES[fp]}\n" for c in W.vortex_batch_configs(n=64, seed=seed, cfl=0.05)]
     cells = sum(W.parse_config(c).nx * W.parse_config(c).ny for c in cfgs)
     log(f"batch: 64 vortex solves, {cells} cells; autotune")
     b = W.Batch(cfgs)
@@ -157,14 +169,14 @@ def batched(W, steps, seq_steps, seed=0):
             "sequential_value": seq, "sequential_steps": seq_steps, "speedup_vs_sequential": value / seq}
 
 
-def hybrid(W, n, seed, t_end, truth):
+def hybrid(W, n, seed, t_end, truth, fp):
     """Learned switch against plain WENO-Z on an unseen seeded four-quadrant
     problem (seed outside the training seeds 0..255), RK3 CFL 0.6 to t_end,
     both timed over the whole run. With truth, both are also compared with
     plain WENO-Z on the nested (2n-1)^2 grid sampled at even indices."""
     P, thr = W.load_switch()
     cfg = W.config_text(TESTCASE=5, SCHEME=1, RKMETH=2, CFL=0.6, ELEMNUMX=n, ELEMNUMY=n, QUAD_SEED=seed,
-                        TEND="%.17g" % t_end)
+                        TEND="%.17g" % t_end, FP_CONTRACT=FP_MODES[fp])
     res = {}
     for name, mode in (("plain", W.SWITCH_OFF), ("hybrid", W.SWITCH_MLP)):
         log(f"hybrid comparison: {name} {n}x{n} seed {seed} to t={t_end}")
@@ -199,7 +211,7 @@ def hybrid(W, n, seed, t_end, truth):
         m = 2 * n - 1
         log(f"hybrid comparison: truth {m}x{m}")
         cfg_t = W.config_text(TESTCASE=5, SCHEME=1, RKMETH=2, CFL=0.6, ELEMNUMX=m, ELEMNUMY=m, QUAD_SEED=seed,
-                              TEND="%.17g" % t_end)
+                              TEND="%.17g" % t_end, FP_CONTRACT=FP_MODES[fp])
         with W.Solver(cfg_t) as s:
             s.advance(0)
             rt = s.get_state()[::2, ::2, 0]
@@ -223,6 +235,10 @@ def main():
     ap.add_argument("--case", default="vortex", choices=["vortex", "quadrant"],
                     help="isentropic vortex (the headline) or a seeded four-quadrant problem")
     ap.add_argument("--quad-seed", type=int, default=0)
+    ap.add_argument("--fp", default="fused", choices=list(FP_MODES),
+                    help="FP64 arithmetic of the sweeps: fused (FMA contraction) or exact (none)")
+    ap.add_argument("--other-steps", type=int, default=100,
+                    help="steps timed in the other FP mode for comparison (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for cpu_baseline (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=200, help="steps of the 64-solve batch (0: skip)")
     ap.add_argument("--seq-steps", type=int, default=20, help="steps per solve of the sequential comparison")
@@ -238,21 +254,36 @@ def main():
     import wenofd as W
 
     n, sc = args.size, SCHEMES[args.scheme]
-    log(f"{args.case} {n}x{n} WENO-{args.scheme} RK3 CFL {args.cfl}: create")
-    if args.case == "vortex":
-        s = W.Solver(TESTCASE=4, SCHEME=sc, RKMETH=2, CFL=args.cfl, ELEMNUMX=n, ELEMNUMY=n, TEND=1e9)
-    else:
-        s = W.Solver(TESTCASE=5, SCHEME=sc, RKMETH=2, CFL=args.cfl, ELEMNUMX=n, ELEMNUMY=n, TEND=1e9,
-                     QUAD_SEED=args.quad_seed)
-    log(f"warmup {args.warmup} steps")
-    s.advance(args.warmup)
-    s.synchronize()
-    log(f"timed {args.steps} steps")
-    t0 = time.perf_counter()
-    done = s.advance(args.steps)  # the device loop: dt folded and applied on the GPU, graph replays
-    s.synchronize()
-    el = time.perf_counter() - t0
-    assert done == args.steps
+
+    def make_solver(fp):
+        keys = dict(SCHEME=sc, RKMETH=2, CFL=args.cfl, ELEMNUMX=n, ELEMNUMY=n, TEND=1e9, FP_CONTRACT=FP_MODES[fp])
+        if args.case == "vortex":
+            return W.Solver(TESTCASE=4, **keys)
+        return W.Solver(TESTCASE=5, QUAD_SEED=args.quad_seed, **keys)
+
+    def timed_run(s, warmup, steps):
+        s.advance(warmup)
+        s.synchronize()
+        t0 = time.perf_counter()
+        done = s.advance(steps)  # the device loop: dt folded and applied on the GPU, graph replays
+        s.synchronize()
+        el = time.perf_counter() - t0
+        assert done == steps
+        return el
+
+    modes = {}
+    other = [m for m in FP_MODES if m != args.fp][0]
+    if args.other_steps > 0:
+        log(f"{args.case} {n}x{n} WENO-{args.scheme} RK3 CFL {args.cfl}, {other} FP64: {args.other_steps} steps")
+        with make_solver(other) as so:
+            el_o = timed_run(so, args.warmup, args.other_steps)
+        modes[other] = {"value": n * n * args.other_steps / el_o, "ms_per_step": el_o * 1e3 / args.other_steps,
+                        "steps": args.other_steps}
+    log(f"{args.case} {n}x{n} WENO-{args.scheme} RK3 CFL {args.cfl}, {args.fp} FP64: warmup {args.warmup}, "
+        f"timed {args.steps} steps")
+    s = make_solver(args.fp)
+    el = timed_run(s, args.warmup, args.steps)
+    modes[args.fp] = {"value": n * n * args.steps / el, "ms_per_step": el * 1e3 / args.steps, "steps": args.steps}
     # per-kernel durations: HIP events around each launch of host-driven steps
     # (the same tile functions the device loop runs), outside the timed region
     ksteps = min(args.steps, 50)
@@ -293,7 +324,10 @@ def main():
         "config": {"workload": (f"isentropic vortex {n}x{n}" if args.case == "vortex" else
                                 f"seeded four-quadrant problem (seed {args.quad_seed}) {n}x{n}") +
                                f", WENO-{args.scheme}, RK3, CFL {args.cfl}, dt from calculateDeltaT every step",
-                   "nx": n, "ny": n, "scheme": args.scheme, "rk": "RK3", "cfl": args.cfl, "case": args.case},
+                   "nx": n, "ny": n, "scheme": args.scheme, "rk": "RK3", "cfl": args.cfl, "case": args.case,
+                   "fp": args.fp + (" (FP_CONTRACT = 1: FMA contraction, as the reference's -O3 -march=native "
+                                    "build)" if args.fp == "fused" else " (FP_CONTRACT = 0: no contraction)")},
+        "fp_modes": modes,
         "roofline": {
             "kernel": dom,
             "bound": "fp64",
@@ -301,17 +335,18 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": dk["tflops"] / FP64_PEAK_TFLOPS,
-            "traffic": measured_traffic(dom, n) if (args.scheme == "Z" and args.case == "vortex") else None,
+            "traffic": measured_traffic(dom, n, args.fp) if (args.scheme == "Z" and args.case == "vortex") else None,
             "hbm": {"achieved": dk["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dk["gbs"] / HBM_PEAK_GBS},
-            "valu_issue": valu_issue(dom, n, dk["avg_us"]) if (args.scheme == "Z" and args.case == "vortex") else None,
+            "valu_issue": (valu_issue(dom, n, dk["avg_us"], args.fp)
+                           if (args.scheme == "Z" and args.case == "vortex") else None),
             "step_fp64_tflops": step_flops / (el / args.steps) / 1e12,
             "kernels": kernels,
         },
     }
     if args.batch_steps > 0:
-        rec["batched"] = batched(W, args.batch_steps, args.seq_steps)
+        rec["batched"] = batched(W, args.batch_steps, args.seq_steps, args.fp)
     if args.hybrid_size > 0:
-        rec["hybrid"] = hybrid(W, args.hybrid_size, args.hybrid_seed, args.hybrid_tend, args.hybrid_truth)
+        rec["hybrid"] = hybrid(W, args.hybrid_size, args.hybrid_seed, args.hybrid_tend, args.hybrid_truth, args.fp)
     if args.cpu_steps > 0:
         log(f"cpu baseline: {args.cpu_steps} oracle steps at {n}x{n}")
         rec["cpu_baseline"] = cpu_baseline(n, n, sc, args.cfl, args.cpu_steps)

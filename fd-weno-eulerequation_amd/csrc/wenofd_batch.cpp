@@ -86,9 +86,10 @@ int stage(DevLoop& L, const BatchView& v, int in, int out, int mode, double a, d
           hipStream_t s) {
     int rc;
     if ((rc = check_launch(launch_batch_ghost(v, in, L.tab_ghost, L.n_ghost, s)))) return rc;
-    if ((rc = check_launch(launch_batch_flux_y(L.scheme, L.tiles, v, in, L.tab_y, L.n_y, s)))) return rc;
+    if ((rc = check_launch(WENOFD_FP(L.fp, launch_batch_flux_y(L.scheme, L.tiles, v, in, L.tab_y, L.n_y, s)))))
+        return rc;
     return check_launch(
-        launch_batch_flux_x_rk(L.scheme, L.tiles, v, in, out, mode, a, bc, c, last, L.tab_x, L.n_x, s));
+        WENOFD_FP(L.fp, launch_batch_flux_x_rk(L.scheme, L.tiles, v, in, out, mode, a, bc, c, last, L.tab_x, L.n_x, s)));
 }
 
 // One iteration of the run() loop for every solve (src/WENOFD.cpp:213-239):
@@ -181,8 +182,8 @@ int devloop_tables(DevLoop& L, std::vector<SolveDesc>& desc, SolveDesc* desc_d, 
     return 0;
 }
 
-int devloop_prime(const BatchView& v, hipStream_t s) {
-    BHIP_TRY(launch_batch_dt_partial(v, s));
+int devloop_prime(const BatchView& v, int fp, hipStream_t s) {
+    BHIP_TRY(WENOFD_FP(fp, launch_batch_dt_partial(v, s)));
     BHIP_TRY(launch_batch_advance(v, true, s));
     return 0;
 }
@@ -204,7 +205,7 @@ namespace {
 
 int build_tables(wenofd_batch* b) { return devloop_tables(b->loop, b->desc, b->desc_d, b->part_cap); }
 
-int prime(wenofd_batch* b) { return devloop_prime(b->view(), b->stream); }
+int prime(wenofd_batch* b) { return devloop_prime(b->view(), b->loop.fp, b->stream); }
 
 int enqueue_steps(wenofd_batch* b, int n) { return devloop_steps(b->loop, b->view(), n, b->stream); }
 
@@ -265,8 +266,9 @@ int create_impl(const char* const* texts, int n, int device, wenofd_batch** out)
         if (s == 0) {
             b->scheme = b->loop.scheme = o.scheme;
             b->rkmeth = b->loop.rkmeth = o.rkmeth;
-        } else if (o.scheme != b->scheme || o.rkmeth != b->rkmeth) {
-            return bail(fail(WENOFD_E_CONFIG, "all solves of a batch must share SCHEME and RKMETH"));
+            b->loop.fp = o.fp_contract;
+        } else if (o.scheme != b->scheme || o.rkmeth != b->rkmeth || o.fp_contract != b->loop.fp) {
+            return bail(fail(WENOFD_E_CONFIG, "all solves of a batch must share SCHEME, RKMETH and FP_CONTRACT"));
         }
         make_case(o, b->cs[s]);
         make_grid(b->cs[s], o.nx, o.ny, b->grid[s]);

@@ -180,13 +180,16 @@ int stage(wenofd_solver* h, double* in, const double* un, double* out, const RkC
         (rc = timed(h, WENOFD_K_GHOST, [&] { return launch_ghost_fill(in, h->special, h->g, h->bc, h->stream); })))
         return rc;
     if ((rc = timed(h, WENOFD_K_FLUX_Y, [&] {
-             return launch_flux_y(h->opt.scheme, h->tiles, in, h->Fy, h->g, h->stream, sw ? h->fl_y : nullptr);
+             return WENOFD_FP(h->opt.fp_contract,
+                              launch_flux_y(h->opt.scheme, h->tiles, in, h->Fy, h->g, h->stream, sw ? h->fl_y : nullptr));
          })))
         return rc;
     if ((rc = timed(h, WENOFD_K_FLUX_X, [&] {
-             return launch_flux_x_rk(h->opt.scheme, h->tiles, in, h->Fy, un, out, rhs_out, h->g, k, h->opt.cfl,
-                                     last_stage ? h->dt_part : nullptr, last_stage ? h->dt_part_nan : nullptr,
-                                     h->stream, sw ? h->fl_x : nullptr);
+             return WENOFD_FP(h->opt.fp_contract,
+                              launch_flux_x_rk(h->opt.scheme, h->tiles, in, h->Fy, un, out, rhs_out, h->g, k,
+                                               h->opt.cfl, last_stage ? h->dt_part : nullptr,
+                                               last_stage ? h->dt_part_nan : nullptr, h->stream,
+                                               sw ? h->fl_x : nullptr));
          })))
         return rc;
     if (last_stage) h->dt_parts = (int)flux_x_rk_blocks(h->g, h->tiles);
@@ -293,7 +296,8 @@ int delta_t(wenofd_solver* h, double* dt) {
                    [&] { return launch_dt_final(h->dt_part, h->dt_part_nan, n, h->dt_dev, h->stream); });
     } else {
         rc = timed(h, WENOFD_K_DT, [&] {
-            return launch_dt(h->U, h->g, h->opt.cfl, h->dt_part, h->dt_part_nan, kDtBlocks, h->dt_dev, h->stream);
+            return WENOFD_FP(h->opt.fp_contract, launch_dt(h->U, h->g, h->opt.cfl, h->dt_part, h->dt_part_nan,
+                                                           kDtBlocks, h->dt_dev, h->stream));
         });
     }
     if (rc) return rc;
@@ -472,6 +476,7 @@ int loop_setup(wenofd_solver* h) {
     HIP_TRY(hipMemcpy(h->loop_desc_d, &d, sizeof(SolveDesc), hipMemcpyHostToDevice));
     devloop_drop_graph(h->loop);
     h->loop.tiles = h->tiles;
+    h->loop.fp = h->opt.fp_contract;
     h->loop_ready = true;
     return 0;
 }
@@ -490,10 +495,12 @@ int enqueue_device_step(wenofd_solver* h, const BatchView& v) {
     auto stage_dev = [&](double* in, const double* un, double* out, const RkCoef& k, bool last) -> int {
         int rc;
         if ((rc = chk(launch_ghost_fill(in, h->special, h->g, h->bc, h->stream, st)))) return rc;
-        if ((rc = chk(launch_flux_y(h->opt.scheme, h->tiles, in, h->Fy, h->g, h->stream, nullptr, st)))) return rc;
-        return chk(launch_flux_x_rk(h->opt.scheme, h->tiles, in, h->Fy, un, out, nullptr, h->g, k, h->opt.cfl,
-                                    last ? h->dt_part : nullptr, last ? h->dt_part_nan : nullptr, h->stream, nullptr,
-                                    st));
+        const int fp = h->opt.fp_contract;
+        if ((rc = chk(WENOFD_FP(fp, launch_flux_y(h->opt.scheme, h->tiles, in, h->Fy, h->g, h->stream, nullptr, st)))))
+            return rc;
+        return chk(WENOFD_FP(fp, launch_flux_x_rk(h->opt.scheme, h->tiles, in, h->Fy, un, out, nullptr, h->g, k,
+                                                  h->opt.cfl, last ? h->dt_part : nullptr,
+                                                  last ? h->dt_part_nan : nullptr, h->stream, nullptr, st)));
     };
     int rc;
     if ((rc = chk(launch_batch_dt(v, h->stream)))) return rc;
@@ -560,7 +567,7 @@ int advance_on_device(wenofd_solver* h, int max_steps, int* steps_done) {
     HIP_TRY(hipMemcpyAsync(h->loop_st_d, h->loop_st_h, sizeof(SolveState), hipMemcpyHostToDevice, h->stream));
     if (h->dt_parts > 0) {  // the last stage's partials describe the current state
         HIP_TRY(launch_batch_advance(v, true, h->stream));
-    } else if ((rc = devloop_prime(v, h->stream))) {
+    } else if ((rc = devloop_prime(v, h->opt.fp_contract, h->stream))) {
         return rc;
     }
     const long long steps0 = h->steps;

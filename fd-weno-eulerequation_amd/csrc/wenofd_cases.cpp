@@ -123,6 +123,13 @@ int interpret_options(const std::map<std::string, std::string>& kv, Options& o, 
         if (e == b) { err = "QUAD_SEED is not an integer"; return WENOFD_E_CONFIG; }
         o.has_quad_seed = true;
     }
+    auto ifp = kv.find("FP_CONTRACT");
+    if (ifp != kv.end()) {
+        long v;
+        if (!leading_int(ifp->second, v)) { err = "FP_CONTRACT is not an integer"; return WENOFD_E_CONFIG; }
+        if (v != 0 && v != 1) { err = "FP_CONTRACT must be 0 (exact) or 1 (FMA contraction)"; return WENOFD_E_BAD_ENUM; }
+        o.fp_contract = (int)v;
+    }
     auto io = kv.find("OUTPUTDIR");
     if (io != kv.end() && !io->second.empty()) {
         o.output_dir = io->second;

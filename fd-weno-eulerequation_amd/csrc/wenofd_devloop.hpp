@@ -17,6 +17,7 @@ namespace wenofd {
 
 struct DevLoop {
     int scheme = 0, rkmeth = 2;
+    int fp = kFpExact;  // FpMode of the sweeps
     Tiles tiles{};
     TileRef* tab_ghost = nullptr;  // block -> (solve, tile) of the three per-stage launches
     TileRef* tab_y = nullptr;
@@ -32,7 +33,7 @@ struct DevLoop {
 int devloop_tables(DevLoop& L, std::vector<SolveDesc>& desc, SolveDesc* desc_d, size_t part_cap);
 // CFL minimum of every solve's current state into its partials, and the count
 // of running solves (the state a run starts from).
-int devloop_prime(const BatchView& v, hipStream_t s);
+int devloop_prime(const BatchView& v, int fp, hipStream_t s);
 // Enqueues n steps (whole graph replays, then plain steps for the remainder).
 int devloop_steps(DevLoop& L, const BatchView& v, int n, hipStream_t s);
 void devloop_drop_graph(DevLoop& L);

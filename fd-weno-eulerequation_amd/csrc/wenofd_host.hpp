@@ -23,6 +23,7 @@ struct Options {
     double quad_states[4][4] = {};
     bool has_quad_seed = false;
     uint64_t quad_seed = 0;
+    int fp_contract = 0;  // FP_CONTRACT: FP64 arithmetic mode of the sweeps (FpMode)
     std::string output_dir = "./output/";
 };
 

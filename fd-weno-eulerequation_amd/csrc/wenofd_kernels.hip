@@ -13,6 +13,9 @@
 // Every phase is a tile function shared by two kernels: one solve per launch
 // (blockIdx = tile), and a batch of independent solves per launch (a table
 // maps each block to a solve and a tile of it, so uneven grids share a launch).
+//
+// The file is compiled twice: -ffp-contract=off -DWENOFD_FP_NS=exact and
+// -ffp-contract=fast -DWENOFD_FP_NS=fused (FpMode in wenofd_kernels.hpp).
 
 #include <hip/hip_runtime.h>
 
@@ -20,7 +23,12 @@
 #include "wenofd_math.hpp"
 #include "wenofd_switch.hpp"
 
+#ifndef WENOFD_FP_NS
+#error "build with -DWENOFD_FP_NS=exact (with -ffp-contract=off) or -DWENOFD_FP_NS=fused (with -ffp-contract=fast)"
+#endif
+
 namespace wenofd {
+namespace WENOFD_FP_NS {
 
 // ---------------------------------------------------------------------------
 // Ghost fill: exchangeGhostCellsValue (one-rank periodic-x ring) + setBoundary,
@@ -857,4 +865,5 @@ hipError_t launch_batch_copy(const Tiles& t, const BatchView& b, int from, int t
     return hipGetLastError();
 }
 
+}  // namespace WENOFD_FP_NS
 }  // namespace wenofd

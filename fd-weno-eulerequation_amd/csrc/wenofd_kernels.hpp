@@ -36,39 +36,10 @@ struct RkCoef {
 struct Tiles {
     int x = 256, y = 64;
 };
-bool tiles_valid(const Tiles& t);
 
 constexpr int kProbePhysStride = 82;
 
-struct SolveState;
-
-// ---- one solve --------------------------------------------------------------
-// st (optional, device): the solve's time state on the device loop; the
-// launches do nothing for an inactive solve and the x sweep takes its step
-// as k.cdt * st->dt.
-hipError_t launch_ghost_fill(double* U, const double* special, const Geom& g, const Bc& bc, hipStream_t s,
-                             const SolveState* st = nullptr);
-hipError_t launch_copy_interior(const double* from, double* to, const Geom& g, const SolveState* st, hipStream_t s);
-// flags (optional): per-face switch flags of the direction (x faces
-// [ny][nx+1], y faces [ny+1][nx]); zero selects the upwind-5 flux.
-hipError_t launch_flux_y(int scheme, const Tiles& t, const double* U, double* Fy, const Geom& g, hipStream_t s,
-                         const unsigned char* flags = nullptr, const SolveState* st = nullptr);
-// x faces + rhs + RK stage. dt_part (optional) receives one partial CFL
-// minimum per block: flux_x_rk_blocks(g, t) entries.
-hipError_t launch_flux_x_rk(int scheme, const Tiles& t, const double* Uin, const double* Fy, const double* Un,
-                            double* Uout, double* rhs_out, const Geom& g, const RkCoef& k, double cfl,
-                            double* dt_part, int* dt_nan, hipStream_t s, const unsigned char* flags = nullptr,
-                            const SolveState* st = nullptr);
-long flux_x_rk_blocks(const Geom& g, const Tiles& t);
-long flux_y_blocks(const Geom& g, const Tiles& t);
-long ghost_blocks(const Geom& g);
-hipError_t launch_dt_final(const double* part, const int* part_nan, int nparts, double* out, hipStream_t s);
-hipError_t launch_dt(const double* U, const Geom& g, double cfl, double* part, int* part_nan, int nblocks,
-                     double* out, hipStream_t s);
-hipError_t launch_probe_weno(int scheme, const double* v5, int n, double zp_lambda, double* out, hipStream_t s);
-hipError_t launch_probe_physics(const double* U, int n, double gamma, double* out, hipStream_t s);
-
-// ---- many solves, one launch per phase ----------------------------------------
+// ---- many solves: per-solve tables on the device ------------------------------
 // Per-solve constants (device array, one entry per solve).
 struct SolveDesc {
     Geom g;
@@ -96,9 +67,6 @@ struct SolveState {
 struct TileRef {
     int solve, tile, bx, by;
 };
-// Tiles per row of the two sweeps (flux_x_rk_blocks = x_tiles_per_row * ny).
-int x_tiles_per_row(const Geom& g, const Tiles& t);
-int y_tiles_per_row(const Geom& g, const Tiles& t);
 
 struct BatchView {
     const SolveDesc* desc;
@@ -112,23 +80,49 @@ struct BatchView {
     int nsolves;
 };
 
-hipError_t launch_batch_ghost(const BatchView& b, int buf, const TileRef* tiles, int nblocks, hipStream_t s);
-hipError_t launch_batch_flux_y(int scheme, const Tiles& t, const BatchView& b, int buf, const TileRef* tiles,
-                               int nblocks, hipStream_t s);
-// Stage in -> out with Un = U[0]; each solve's cdt is c_stage * its dt.
-hipError_t launch_batch_flux_x_rk(int scheme, const Tiles& t, const BatchView& b, int in, int out, int mode,
-                                  double a, double bcoef, double c_stage, bool last, const TileRef* tiles,
-                                  int nblocks, hipStream_t s);
-hipError_t launch_batch_dt_partial(const BatchView& b, hipStream_t s);
-// dt of every running solve from its partials (clipped to t_end - now) and
-// the step's activity flags; after the stages, the time advance of the active
-// solves and the count of solves still running (count_only: the count alone).
-hipError_t launch_batch_dt(const BatchView& b, hipStream_t s);
-hipError_t launch_batch_advance(const BatchView& b, bool count_only, hipStream_t s);
-// Interior copy U[from] -> U[to] of the active solves (forward-Euler batches).
-// The tile table is the one of the fused x kernel for the same tile shape.
-hipError_t launch_batch_copy(const Tiles& t, const BatchView& b, int from, int to, const TileRef* tiles, int nblocks,
-                             hipStream_t s);
+// ---- launchers -------------------------------------------------------------------
+// The FP64 sweeps exist in two arithmetic modes, built from the same source:
+//   exact: no FMA contraction; every product and sum is rounded as the
+//          reference writes it (bit-identical to the reference compiled
+//          without contraction);
+//   fused: the compiler contracts a * b + c into one FMA where it finds one
+//          (one rounding instead of two, as the reference's own optimised CPU
+//          build does with -march=native); fewer instructions.
+enum FpMode { kFpExact = 0, kFpFused = 1 };
+
+namespace exact {
+#include "wenofd_sweeps.inc"
+}
+namespace fused {
+#include "wenofd_sweeps.inc"
+}
+
+// Launches whose result contraction cannot change (copies, sign flips,
+// minima, clock updates, block counts) and the test probes come from the
+// exact build. (Not inside wenofd_kernels.hip itself, where argument-dependent
+// lookup would see them beside the build's own definitions.)
+#ifndef WENOFD_FP_NS
+using exact::tiles_valid;
+using exact::launch_ghost_fill;
+using exact::launch_copy_interior;
+using exact::flux_x_rk_blocks;
+using exact::flux_y_blocks;
+using exact::ghost_blocks;
+using exact::launch_dt_final;
+using exact::launch_probe_weno;
+using exact::launch_probe_physics;
+using exact::x_tiles_per_row;
+using exact::y_tiles_per_row;
+using exact::launch_batch_ghost;
+using exact::launch_batch_dt;
+using exact::launch_batch_advance;
+using exact::launch_batch_copy;
+#endif
+
+// The launchers with FP64 arithmetic (launch_flux_y, launch_flux_x_rk,
+// launch_dt, launch_batch_flux_y, launch_batch_flux_x_rk,
+// launch_batch_dt_partial) are called in a mode: WENOFD_FP(mode, launch_...(...)).
+#define WENOFD_FP(mode, call) ((mode) == ::wenofd::kFpFused ? ::wenofd::fused::call : ::wenofd::exact::call)
 
 // ---- learned switch (wenofd_switch.hip) ----------------------------------------
 // Flags (and optionally logits [faces][4]) of every face of direction dir from

@@ -80,7 +80,14 @@ int wenofd_version(void);
  * 2 Riemann2, 3 RTI, 4 isentropic vortex, 5 four-quadrant), SCHEME (0 JS, 1 Z,
  * 2 ZP, 3 ZPI), RKMETH (0 RK1, 1 RK2, 2 RK3), CFL, ELEMNUMX, ELEMNUMY.
  * Optional: TEND, VORTEX_EPS/VORTEX_XC/VORTEX_YC, QUAD_STATES (16 numbers),
- * QUAD_SEED, OUTPUTDIR. device < 0 uses the current device.
+ * QUAD_SEED, OUTPUTDIR, FP_CONTRACT. device < 0 uses the current device.
+ * FP_CONTRACT (no reference counterpart) selects the FP64 arithmetic of the
+ * flux sweeps, the RK update and the CFL step: 0 (default) rounds every
+ * product and sum as the reference's source writes it, so results equal the
+ * reference built without FMA contraction bit for bit; 1 lets the compiler
+ * contract a * b + c into FMAs, as the reference's optimised CPU build does,
+ * for fewer instructions (results then agree with either reference build to
+ * the few-ulp level of those two builds' own difference).
  */
 int wenofd_create(const char* config_text, int device, wenofd_solver** out);
 /* CConfig(path).read(option) then wenofd_create (utils/readConfig.hpp:39-80). */
@@ -227,7 +234,7 @@ int wenofd_mlp_evaluate(wenofd_mlp* m, long long start, long long n, float thr, 
  * Batched solves. No counterpart in the reference, which runs one case per
  * process: a batch holds many independent CWENOFD runs (each its own config
  * text: grid size, test case, CFL, TEND, vortex parameters ...; all sharing
- * SCHEME and RKMETH) and advances them together, one kernel launch per phase
+ * SCHEME, RKMETH and FP_CONTRACT) and advances them together, one kernel launch per phase
  * for the whole batch. Each solve's time step is computed and applied on the
  * device (calculateDeltaT, src/WENOFD.cpp:398-452, clipped to its own t_end);
  * a solve that reaches its t_end (|now - t_end| <= 1e-9, :213) or meets a NaN

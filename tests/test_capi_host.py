@@ -53,7 +53,7 @@ def test_parse_config_values_use_numeric_prefix():
 @pytest.mark.parametrize("key,val,code", [
     ("TESTCASE", "7", W.E_BAD_ENUM), ("SCHEME", "4", W.E_BAD_ENUM), ("RKMETH", "3", W.E_BAD_ENUM),
     ("SCHEME", "-1", W.E_BAD_ENUM), ("CFL", "abc", W.E_CONFIG), ("ELEMNUMX", "3", W.E_CONFIG),
-    ("ELEMNUMY", "", W.E_CONFIG),
+    ("ELEMNUMY", "", W.E_CONFIG), ("FP_CONTRACT", "2", W.E_BAD_ENUM), ("FP_CONTRACT", "x", W.E_CONFIG),
 ])
 def test_parse_config_errors(key, val, code):
     keys = dict(TESTCASE="0", SCHEME="1", RKMETH="2", CFL="0.5", ELEMNUMX="32", ELEMNUMY="32")
@@ -93,12 +93,14 @@ def test_create_errors_before_touching_a_device():
 
 
 def test_batch_config_errors_before_touching_a_device():
-    """A batch shares SCHEME and RKMETH; a bad member config names its index."""
+    """A batch shares SCHEME, RKMETH and FP_CONTRACT; a bad member config names its index."""
     a = W.config_text(TESTCASE=4, SCHEME=1, RKMETH=2, CFL=0.5, ELEMNUMX=16, ELEMNUMY=16)
     b = W.config_text(TESTCASE=4, SCHEME=0, RKMETH=2, CFL=0.5, ELEMNUMX=16, ELEMNUMY=16)
     bad = W.config_text(TESTCASE=4, SCHEME=1, RKMETH=2, CFL=0.5, ELEMNUMX=3, ELEMNUMY=16)
+    fused = a + "FP_CONTRACT = 1\n"
     h = ctypes.c_void_p()
-    for texts, code, text in (([a, b], W.E_CONFIG, "share SCHEME"), ([a, bad], W.E_CONFIG, "solve 1")):
+    for texts, code, text in (([a, b], W.E_CONFIG, "share SCHEME"), ([a, bad], W.E_CONFIG, "solve 1"),
+                              ([a, fused], W.E_CONFIG, "FP_CONTRACT")):
         arr = (ctypes.c_char_p * len(texts))(*[t.encode() for t in texts])
         rc = W.lib().wenofd_batch_create(arr, len(texts), 0, ctypes.byref(h))
         assert rc == code and not h.value
