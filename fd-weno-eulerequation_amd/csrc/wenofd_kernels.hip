@@ -109,12 +109,14 @@ __global__ __launch_bounds__(kGhostThreads) void k_ghost_fill(double* __restrict
 // y faces. One tile = 64 interior columns x YF faces along y, marched in steps
 // of 4 face rows (256 threads = 64 columns x 4 rows). The LLF split of the 9
 // point rows one step reads lives in an LDS ring; each step adds 4 new rows, so
-// every point is split once per tile (plus the 5-row lead-in). Loads of U are
-// coalesced along x.
+// every point is split once per tile (plus the 5-row lead-in). The points
+// themselves stay in a second ring of 8 rows for the faces' mean states. Loads
+// of U are coalesced along x; each point is read from HBM once per tile.
 // ---------------------------------------------------------------------------
 constexpr int kYC = 64;   // columns per tile
 constexpr int kYR = 4;    // face rows per step
 constexpr int kRing = 9;  // ring rows: a step reads 9 consecutive point rows
+constexpr int kRawRing = 8;  // point rows f-1 .. f+3 read while rows f+2 .. f+5 are written
 
 // Block-wide compaction of the flagged faces of one pass: slot `idx` of the
 // returned count receives the thread index of each flagged thread, in thread
@@ -146,6 +148,7 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
     static_assert(YF % kYR == 0, "YF must be a multiple of the step height");
     __shared__ double sP[4][kRing][kYC];
     __shared__ double sM[4][kRing][kYC];
+    __shared__ double sU[4][kRawRing][kYC];
     __shared__ int sIdx[HYB ? kYC * kYR : 1];
     __shared__ int sCnt[kYR];
     const int tx = threadIdx.x % kYC;
@@ -168,6 +171,7 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
             for (int r = 0; r < 4; ++r) {
                 sP[r][slot][tx] = fp[r];
                 sM[r][slot][tx] = fm[r];
+                sU[r][(prow - base) % kRawRing][tx] = u[r];
             }
         }
     };
@@ -178,11 +182,10 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
         auto full_face = [&](int cx, int fr) {
             const int ic = 4 + bx * kYC + cx;
             double Ul[4], Ur[4], Fh[4];
-            const size_t ol = (size_t)(fr - 1) * g.pitch + ic;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                Ul[r] = U[r * g.plane + ol];
-                Ur[r] = U[r * g.plane + ol + g.pitch];
+            for (int r = 0; r < 4; ++r) {  // points fr-1 and fr
+                Ul[r] = sU[r][(fr - 1 - base) % kRawRing][cx];
+                Ur[r] = sU[r][(fr - base) % kRawRing][cx];
             }
             const int s0 = fr - 3 - base;  // ring position of point row fr-3
             face_flux_dev<1, S>(

@@ -299,10 +299,8 @@ WENOFD_HD void project_right(const T R[16], const T vh[4], T F[4]) {
 // sp(s, r) / sm(s, r) return component r of F+ / F- at stencil slot s
 // (slot 0 = point f-3, slot 5 = point f+2).
 template <int D, int S, class T, class GetP, class GetM>
-WENOFD_HD void face_flux(const T Ul[4], const T Ur[4], GetP sp, GetM sm, double gamma, double zp_lambda, T Fh[4]) {
-    T ave[4], L[16], R[16];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) ave[r] = 0.5 * (Ul[r] + Ur[r]);
+WENOFD_HD void face_flux_ave(const T ave[4], GetP sp, GetM sm, double gamma, double zp_lambda, T Fh[4]) {
+    T L[16], R[16];
     eigen_matrices<D>(ave, gamma, L, R);
     T vp[5][4], vm[5][4];
 #pragma unroll
@@ -326,11 +324,21 @@ WENOFD_HD void face_flux(const T Ul[4], const T Ur[4], GetP sp, GetM sm, double 
     project_right<D>(R, vh, Fh);
 }
 
+template <int D, int S, class T, class GetP, class GetM>
+WENOFD_HD void face_flux(const T Ul[4], const T Ur[4], GetP sp, GetM sm, double gamma, double zp_lambda, T Fh[4]) {
+    T ave[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ave[r] = 0.5 * (Ul[r] + Ur[r]);
+    face_flux_ave<D, S>(ave, sp, sm, gamma, zp_lambda, Fh);
+}
+
 // face_flux as the kernels run it: the 8 reconstructions use the shared
 // refined reciprocals (weno5_shared_rcp). If an operand of one of them left the
 // exact range in any lane of the wave, the wave recomputes the face with full
 // IEEE divisions and those lanes take that result; the check is wave-uniform
-// and placed after the fast path so that it keeps no extra registers alive.
+// and placed after the fast path; the recomputation starts from the mean
+// state, so only that (not the face's two points) stays live across the fast
+// path.
 template <int D, int S, class GetP, class GetM>
 __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double Ur[4], GetP sp, GetM sm, double gamma,
                                               double zp_lambda, double Fh[4]) {
@@ -361,7 +369,7 @@ __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double U
     project_right<D>(R, vh, Fh);
     if (__any(!ok)) {
         double Fe[4];
-        face_flux<D, S>(Ul, Ur, sp, sm, gamma, zp_lambda, Fe);
+        face_flux_ave<D, S>(ave, sp, sm, gamma, zp_lambda, Fe);
         if (!ok) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) Fh[r] = Fe[r];
