@@ -22,6 +22,7 @@ import json
 import os
 import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -114,9 +115,52 @@ def valu_issue(kernel, n, avg_us, fp):
             "unit": "wave-instructions/s", "frac": achieved / VALU_PEAK_WAVE_INSTR}
 
 
-def cpu_baseline(nx, ny, scheme, cfl, steps):
-    """The CPU oracle (a from-scratch serial restatement of the reference's
-    arithmetic) timed on this host over a bounded sample of the same workload."""
+REF_EXE = os.path.join(ROOT, "oracle", "_ref", "ref_native")
+MPIEXEC = "/opt/conda/bin/mpiexec"
+
+
+def _host_cpu():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _ref_bench(cmd, workdir):
+    """Runs the reference build's bench mode; returns its stepping seconds."""
+    out = subprocess.run(cmd, cwd=workdir, capture_output=True, text=True, timeout=300, check=True).stdout
+    for line in out.splitlines():
+        if line.startswith("ranks "):
+            return float(line.split()[5])
+    raise RuntimeError("no timing line from the reference build")
+
+
+def cpu_baseline(nx, ny, scheme, cfl, steps, mpi_ranks):
+    """The reference's own CPU path (its sources compiled with the makefile's
+    -O3 -march=native into oracle/_ref/ref_native, driven through its class
+    surface) timed on this host over a bounded sample of the same workload,
+    serially and under mpiexec; the serial CPU oracle when that build is
+    absent."""
+    sname = [k for k, v in SCHEMES.items() if v == scheme][0]
+    if os.path.exists(REF_EXE):
+        args = ["bench", "4", str(scheme), "2", "%.17g" % cfl, str(nx), str(ny), str(steps)]
+        try:
+            with tempfile.TemporaryDirectory() as wd:  # the reference writes ./output/ at start-up
+                el = _ref_bench([REF_EXE] + args, wd)
+                rec = {"value": nx * ny * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                       "sample": f"{steps} steps (calculateDeltaT + RunRK3) of the same {nx}x{ny} vortex, WENO-"
+                                 f"{sname}, the reference's CWENOFD built with its makefile flags, one MPI rank, "
+                                 f"{el:.1f} s", "host": _host_cpu()}
+                if mpi_ranks > 1 and os.path.exists(MPIEXEC):
+                    el_m = _ref_bench([MPIEXEC, "-n", str(mpi_ranks), REF_EXE] + args, wd)
+                    rec["mpi"] = {"ranks": mpi_ranks, "cores": mpi_ranks, "value": nx * ny * steps / el_m,
+                                  "seconds": el_m}
+                return rec
+        except (subprocess.SubprocessError, RuntimeError, OSError, ValueError, IndexError) as e:
+            log(f"cpu baseline: reference build failed ({e}); timing the oracle instead")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -127,7 +171,7 @@ def cpu_baseline(nx, ny, scheme, cfl, steps):
     el = time.perf_counter() - t0
     return {"value": nx * ny * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
             "sample": f"{steps} steps (calculateDeltaT + RunRK3) of the same {nx}x{ny} vortex, WENO-"
-                      f"{[k for k, v in SCHEMES.items() if v == scheme][0]}, serial C++ oracle, {el:.1f} s"}
+                      f"{sname}, serial C++ oracle, {el:.1f} s", "host": _host_cpu()}
 
 
 def batched(W, steps, seq_steps, fp, seed=0):
@@ -243,7 +287,8 @@ def main():
                     help="FP64 arithmetic of the sweeps: fused (FMA contraction) or exact (none)")
     ap.add_argument("--other-steps", type=int, default=100,
                     help="steps timed in the other FP mode for comparison (0: skip)")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for cpu_baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="reference CPU steps for cpu_baseline (0: skip)")
+    ap.add_argument("--cpu-ranks", type=int, default=8, help="MPI ranks of the reference's parallel CPU run")
     ap.add_argument("--batch-steps", type=int, default=200, help="steps of the 64-solve batch (0: skip)")
     ap.add_argument("--seq-steps", type=int, default=20, help="steps per solve of the sequential comparison")
     ap.add_argument("--hybrid-size", type=int, default=2048, help="grid of the learned-switch comparison (0: skip)")
@@ -352,8 +397,8 @@ def main():
     if args.hybrid_size > 0:
         rec["hybrid"] = hybrid(W, args.hybrid_size, args.hybrid_seed, args.hybrid_tend, args.hybrid_truth, args.fp)
     if args.cpu_steps > 0:
-        log(f"cpu baseline: {args.cpu_steps} oracle steps at {n}x{n}")
-        rec["cpu_baseline"] = cpu_baseline(n, n, sc, args.cfl, args.cpu_steps)
+        log(f"cpu baseline: {args.cpu_steps} reference steps at {n}x{n}, serial and {args.cpu_ranks} MPI ranks")
+        rec["cpu_baseline"] = cpu_baseline(n, n, sc, args.cfl, args.cpu_steps, args.cpu_ranks)
     else:
         rec["cpu_baseline"] = None
     print(json.dumps(rec), flush=True)

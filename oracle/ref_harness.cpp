@@ -2,8 +2,9 @@
 // compiled from its own sources where they lie, to write golden vectors.
 //
 // TEST INFRASTRUCTURE ONLY: built by oracle/build_ref.sh into oracle/_ref/
-// (git-ignored), run only by tests/golden/make_golden.py in the container that
-// has the reference mounted. It uses the reference's public class surface
+// (git-ignored), run by tests/golden/make_golden.py in the container that has
+// the reference mounted, and by bench.py (bench mode) to time the reference's
+// CPU path as the baseline. It uses the reference's public class surface
 // (include/WENOFD.hpp:28-124, utils/equations.hpp:55-699) and its inline WENO
 // functions (include/Utility.hpp:415-615). All binary files are little-endian
 // float64; states are interior-only in the reference's [NX][NY][4] order unless
@@ -25,6 +26,12 @@
 //            Works under mpiexec -n P: rank 0 writes the gathered state.
 //   init   same arguments as traj with nsteps = 0 (initial interior state)
 //   run    <tc> <scheme> <rk> <cfl> <nx> <ny>   the reference's CWENOFD::run
+//   bench  <tc> <scheme> <rk> <cfl> <nx> <ny> <nsteps>
+//            as traj from the case's initial state (tc 4 / 5 with the default
+//            parameters), timing only the nsteps iterations of the reference's
+//            loop body (calculateDeltaT, RunRK, now += dt) between barriers with
+//            MPI_Wtime; rank 0 prints "ranks R steps K seconds S" on stdout.
+//            Used by bench.py as the CPU baseline (serial and under mpiexec).
 
 #include <mpi.h>
 
@@ -215,6 +222,36 @@ int main(int argc, char** argv) {
             for (int j = 0; j < ny + 8; ++j)
                 for (int r = 0; r < 4; ++r) full.push_back(s.m_Uh[i][j].vector[r]);
         write_bin(argv[8], full);
+    } else if (mode == "bench") {
+        const int tc = std::atoi(argv[2]), scheme = std::atoi(argv[3]), rk = std::atoi(argv[4]);
+        const double cfl = std::atof(argv[5]);
+        const int nx = std::atoi(argv[6]), ny = std::atoi(argv[7]), nsteps = std::atoi(argv[8]);
+        EulerEquation* eq = new EulerEquation();
+        CWENOFD s(eq);
+        const int base_tc = (tc == 4) ? 0 : (tc == 5 ? 2 : tc);
+        s.initializeSolver(options(base_tc, scheme, rk, cfl, nx, ny));
+        s.m_solverTimer.start();
+        if (tc == 4) {
+            make_vortex(s, eq, 5.0, 5.0, 5.0);
+        } else if (tc == 5) {
+            double q[4][4] = {{1.5, 0, 0, 1.5}, {0.5323, 1.206, 0, 0.3}, {0.138, 1.206, 1.206, 0.029}, {0.5323, 0, 1.206, 0.3}};
+            make_quadrant(s, eq, q);
+        }
+        s.m_outputTime = 1e9;
+        s.initializeAve();
+        s.m_now = 0;
+        MPI_Barrier(MPI_COMM_WORLD);
+        const double t0 = MPI_Wtime();
+        for (int k = 0; k < nsteps; ++k) {
+            const double dt = s.calculateDeltaT();
+            run_rk(s, rk, dt);
+            s.m_now += dt;
+        }
+        MPI_Barrier(MPI_COMM_WORLD);
+        const double t1 = MPI_Wtime();
+        int nranks = 1;
+        MPI_Comm_size(MPI_COMM_WORLD, &nranks);
+        if (rank == 0) std::printf("ranks %d steps %d seconds %.6f\n", nranks, nsteps, t1 - t0);
     } else if (mode == "traj" || mode == "init") {
         const int tc = std::atoi(argv[2]), scheme = std::atoi(argv[3]), rk = std::atoi(argv[4]);
         const double cfl = std::atof(argv[5]);
