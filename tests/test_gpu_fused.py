@@ -165,3 +165,34 @@ def test_fused_device_loop_equals_host_loop(tc, rk):
         a.set_device_loop(0)
         assert a.advance(33) == b.advance(33) == 33
         assert same(a.get_state(), b.get_state()) and a.info().now == b.info().now
+
+
+@pytest.mark.parametrize("mode", ["weno", "linear", "mlp"])
+def test_fused_switched_steps_match_oracle_with_the_same_flags(mode):
+    """The learned switch's sweeps in the contracted mode: each switched step,
+    started from the GPU's own state, agrees with the oracle's hybrid step with
+    the same flags (all WENO, all upwind-5, or the trained network's) to the
+    rounding floor."""
+    import oracle as O
+
+    base = dict(TESTCASE=5, SCHEME=1, RKMETH=2, CFL="0.5", ELEMNUMX=34, ELEMNUMY=29, QUAD_SEED=9, TEND=100,
+                FP_CONTRACT=1)
+    print(f"[gpu] fused switched steps, flags {mode}", flush=True)
+    o = O.Solver(5, 1, 2, 0.5, 34, 29, t_end=100.0, quad=O.seeded_quadrant(9))
+    with W.Solver(**base) as g:
+        if mode == "weno":
+            g.switch_set(W.SWITCH_WENO)
+        elif mode == "linear":
+            g.switch_set(W.SWITCH_LINEAR)
+        else:
+            P, thr = W.load_switch()
+            g.switch_set(W.SWITCH_MLP, P, thr)
+        for _ in range(3):
+            o.set_state(g.get_state())
+            dt = g.step()
+            fx, fy = g.switch_flags()
+            if mode == "mlp":
+                assert 0 < fx.mean() < 1 and 0 < fy.mean() < 1
+            o.set_flags(fx, fy)
+            o.step(dt)
+            assert relmax(g.get_state(), o.get_state()) <= 1e-13
