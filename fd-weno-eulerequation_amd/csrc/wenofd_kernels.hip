@@ -116,7 +116,8 @@ __global__ __launch_bounds__(kGhostThreads) void k_ghost_fill(double* __restrict
 constexpr int kYC = 64;   // columns per tile
 constexpr int kYR = 4;    // face rows per step
 constexpr int kRing = 9;  // ring rows: a step reads 9 consecutive point rows
-constexpr int kRawRing = 8;  // point rows f-1 .. f+3 read while rows f+2 .. f+5 are written
+// Point rows f-1 .. f+3 are read while rows f+2 .. f+5 are written: 8 slots.
+constexpr int kRawRing = 8;
 
 // Block-wide compaction of the flagged faces of one pass: slot `idx` of the
 // returned count receives the thread index of each flagged thread, in thread
@@ -171,7 +172,13 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
             for (int r = 0; r < 4; ++r) {
                 sP[r][slot][tx] = fp[r];
                 sM[r][slot][tx] = fm[r];
-                sU[r][(prow - base) % kRawRing][tx] = u[r];
+            }
+            // Faces read the points from row F0-1 on. The lead-in's rows
+            // F0-3 and F0-2 are not stored: row F0-3 would share its slot
+            // with row F0+5, which the first step writes before any barrier.
+            if (prow - base >= 2) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sU[r][(prow - base) % kRawRing][tx] = u[r];
             }
         }
     };

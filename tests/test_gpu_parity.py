@@ -243,3 +243,22 @@ def test_diagonal_symmetry_2048():
         U = s.get_state()
     T = np.transpose(U, (1, 0, 2))[..., [0, 2, 1, 3]]
     assert relmax(U, T) < 1e-10
+
+
+@pytest.mark.parametrize("fp", [0, 1])
+def test_large_runs_are_repeatable(fp):
+    """At full size the same run gives the same bits every time, on the device
+    loop and on the host loop: no block of a sweep may depend on how its waves
+    are scheduled (a lost race between two writes of one LDS slot shows here
+    as a few differing ulps after some hundred steps)."""
+    keys = dict(TESTCASE=5, SCHEME=1, RKMETH=2, CFL="0.6", ELEMNUMX=2048, ELEMNUMY=2048, QUAD_SEED=1000,
+                FP_CONTRACT=fp)
+    states = []
+    for loop in (1, 1, 0):
+        print(f"[gpu] 2048^2 quadrant, 400 steps, FP_CONTRACT={fp}, device loop {loop}", flush=True)
+        with W.Solver(**keys) as s:
+            s.set_device_loop(loop)
+            assert s.advance(400) == 400
+            states.append(s.get_state())
+    assert same(states[0], states[1]), "two device-loop runs differ"
+    assert same(states[0], states[2]), "device loop and host loop differ"
