@@ -86,7 +86,8 @@ struct wenofd_solver {
     // dilated flags per direction (x faces [ny][nx+1], y faces [ny+1][nx]).
     int sw_mode = WENOFD_SWITCH_OFF;
     float sw_thr = 0.0f;
-    int sw_every = 1;           // classify every sw_every steps, dilation radius sw_every
+    int sw_every = 1;           // classify every sw_every steps
+    int sw_rad = -1;            // dilation radius (-1: sw_every)
     long long sw_since = 0;     // switched steps since the last classification (0: classify now)
     float* sw_params = nullptr;
     unsigned char* fl_raw_x = nullptr;
@@ -209,7 +210,7 @@ int classify(wenofd_solver* h, float* logits_x = nullptr, float* logits_y = null
     if (h->sw_mode != WENOFD_SWITCH_MLP) return 0;
     if (!force && h->sw_since > 0 && h->sw_since < h->sw_every) return 0;
     h->sw_since = 0;
-    const int rad = h->sw_every;
+    const int rad = h->sw_rad >= 0 ? h->sw_rad : h->sw_every;
     return timed(h, WENOFD_K_SWITCH, [&] {
         hipError_t e = launch_classify(0, h->U, h->g, h->sw_params, h->sw_thr, h->fl_raw_x, logits_x, h->stream);
         if (e == hipSuccess)
@@ -1033,6 +1034,14 @@ int wenofd_switch_interval(wenofd_solver* h, int steps) {
     if (!h) return fail(WENOFD_E_ARG, "null handle");
     if (steps < 1 || steps > 16) return fail(WENOFD_E_ARG, "the classification interval must be in [1, 16]");
     h->sw_every = steps;
+    h->sw_since = 0;
+    return 0;
+}
+
+int wenofd_switch_dilation(wenofd_solver* h, int radius) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    if (radius < -1 || radius > 16) return fail(WENOFD_E_ARG, "the dilation radius must be in [0, 16], or -1");
+    h->sw_rad = radius;
     h->sw_since = 0;
     return 0;
 }

@@ -34,7 +34,7 @@ EXPORTS = (
     "wenofd_batch_advance", "wenofd_batch_set_graph_steps", "wenofd_batch_set_tiles", "wenofd_batch_get_tiles",
     "wenofd_batch_autotune", "wenofd_batch_get_state", "wenofd_batch_get_info", "wenofd_batch_accuracy",
     "wenofd_batch_synchronize",
-    "wenofd_switch_set", "wenofd_switch_interval", "wenofd_switch_classify", "wenofd_switch_flags", "wenofd_switch_stats", "wenofd_harvest",
+    "wenofd_switch_set", "wenofd_switch_interval", "wenofd_switch_dilation", "wenofd_switch_classify", "wenofd_switch_flags", "wenofd_switch_stats", "wenofd_harvest",
     "wenofd_mlp_create", "wenofd_mlp_destroy", "wenofd_mlp_set_params", "wenofd_mlp_get_params",
     "wenofd_mlp_dataset_add", "wenofd_mlp_harvest", "wenofd_mlp_dataset_size", "wenofd_mlp_dataset_get",
     "wenofd_mlp_dataset_clear", "wenofd_mlp_grad", "wenofd_mlp_train", "wenofd_mlp_infer", "wenofd_mlp_evaluate",
@@ -114,6 +114,7 @@ def lib():
         L.wenofd_switch_set.argtypes = [vp, i, fp, ctypes.c_float]
         L.wenofd_switch_classify.argtypes = [vp, i, u8, u8, fp, fp]
         L.wenofd_switch_interval.argtypes = [vp, i]
+        L.wenofd_switch_dilation.argtypes = [vp, i]
         L.wenofd_switch_flags.argtypes = [vp, u8, u8]
         L.wenofd_switch_stats.argtypes = [vp, llp]
         L.wenofd_harvest.argtypes = [vp, i, fp, u8]
@@ -332,6 +333,9 @@ class Solver:
 
     def switch_interval(self, steps):
         _check(lib().wenofd_switch_interval(self.h, int(steps)))
+
+    def switch_dilation(self, radius):
+        _check(lib().wenofd_switch_dilation(self.h, int(radius)))
 
     def switch_classify(self, dilate=True, logits=False):
         """(flags_x [NY][NX+1], flags_y [NY+1][NX], logits_x, logits_y) of the current state."""

@@ -189,6 +189,11 @@ int wenofd_switch_classify(wenofd_solver* h, int dilate, unsigned char* flags_x,
 /* Classify every `steps` steps (1..16, default 1) and dilate the flags by
  * `steps` faces: a wave moves less than one cell per step at CFL <= 1. */
 int wenofd_switch_interval(wenofd_solver* h, int steps);
+/* Dilation radius in faces (0..16) in place of the interval; -1 (the
+ * default) follows the interval. A radius below the interval trades the
+ * guarantee above for fewer WENO faces (the plain solver still redoes a step
+ * that fails). No counterpart in the reference. */
+int wenofd_switch_dilation(wenofd_solver* h, int radius);
 /* The (dilated) flags the last switched step used. */
 int wenofd_switch_flags(wenofd_solver* h, unsigned char* flags_x, unsigned char* flags_y);
 /* Number of switched steps the plain solver had to redo. */

@@ -298,6 +298,38 @@ def test_ablation_trained_switch_beats_random_flags_at_the_same_rate():
     assert linear_failed or linear_fallbacks > 0 or e_trained * 20 < e_lin
 
 
+def test_dilation_radius_below_the_interval():
+    """Interval 4 with radius 1: the first step's flags are the raw flags
+    dilated by one face, reused unchanged for four steps, and every switched
+    step still equals the oracle's with those flags."""
+    g, o = pair(5, 40, 34, seed=19)
+    P, thr = W.load_switch()
+    g.switch_set(W.SWITCH_MLP, P, thr)
+    g.switch_interval(4)
+    g.switch_dilation(1)
+    rx, ry, _, _ = g.switch_classify(dilate=False)
+
+    def dil(f, r):
+        p = np.pad(f, r)
+        return np.max([p[r + a:r + a + f.shape[0], r + b:r + b + f.shape[1]]
+                       for a in range(-r, r + 1) for b in range(-r, r + 1)], axis=0)
+
+    seen = []
+    for k in range(5):
+        dt = g.step()
+        fx, fy = g.switch_flags()
+        if k == 0:
+            assert same(fx, dil(rx, 1)) and same(fy, dil(ry, 1))
+        seen.append(fx)
+        o.set_flags(fx, fy)
+        assert o.delta_t() == dt
+        o.step(dt)
+        assert same(g.get_state(), o.get_state())
+    assert all(same(seen[k], seen[0]) for k in range(4))
+    assert not same(seen[4], seen[0])  # reclassified
+    g.close()
+
+
 def test_classification_interval_reuses_dilated_flags():
     g, o = pair(5, 40, 34, seed=19)
     P, thr = W.load_switch()
