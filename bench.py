@@ -10,8 +10,8 @@ from a HIP graph.
 
 FP64 arithmetic mode (--fp): "fused" (the default, config key FP_CONTRACT = 1)
 runs the sweeps built with FMA contraction, like the reference's own
-optimised CPU build; "exact" the build without contraction, bit-identical to
-the reference compiled without it. Both meet the drop-in bar (relative
+optimised CPU build, and division by the reciprocal; "exact" the build
+without either, bit-identical to the reference compiled without contraction. Both meet the drop-in bar (relative
 max-norm <= 1e-11 against the reference), and the line reports both rates.
 
 Prints one JSON line (see the fields below). --gpus > 1 is refused: this
@@ -392,7 +392,8 @@ def main():
                                f", WENO-{args.scheme}, RK3, CFL {args.cfl}, dt from calculateDeltaT every step",
                    "nx": n, "ny": n, "scheme": args.scheme, "rk": "RK3", "cfl": args.cfl, "case": args.case,
                    "fp": args.fp + (" (FP_CONTRACT = 1: FMA contraction, as the reference's -O3 -march=native "
-                                    "build)" if args.fp == "fused" else " (FP_CONTRACT = 0: no contraction)")},
+                                    "build, and division by the reciprocal)" if args.fp == "fused" else
+                                    " (FP_CONTRACT = 0: no contraction)")},
         "fp_modes": modes,
         "roofline": {
             "kernel": dom,

@@ -15,7 +15,8 @@
 // maps each block to a solve and a tile of it, so uneven grids share a launch).
 //
 // The file is compiled twice: -ffp-contract=off -DWENOFD_FP_NS=exact and
-// -ffp-contract=fast -DWENOFD_FP_NS=fused (FpMode in wenofd_kernels.hpp).
+// -ffp-contract=fast -freciprocal-math -DWENOFD_FP_NS=fused -DWENOFD_RECIP_DIV=1
+// (FpMode in wenofd_kernels.hpp).
 
 #include <hip/hip_runtime.h>
 
@@ -236,9 +237,15 @@ __host__ __device__ inline int y_tiles_y(const Geom& g) { return (g.ny + 1 + YF 
 
 // The WENO-ZP y sweep wants a few more registers than 3 waves per SIMD
 // allow; capping it there (a 3-dword spill) runs 2 % faster than 2 waves
-// (for WENO-ZPI the 5-dword spill costs more than it gains).
+// (for WENO-ZPI the 5-dword spill costs more than it gains). With division by
+// the reciprocal the JS and Z sweeps would take 246 VGPRs: capped likewise.
+#if WENOFD_RECIP_DIV
+constexpr bool kCapRecip = true;
+#else
+constexpr bool kCapRecip = false;
+#endif
 template <int S, bool HYB>
-constexpr int y_min_waves() { return S == kZP && !HYB ? 3 : 1; }
+constexpr int y_min_waves() { return !HYB && (S == kZP || (kCapRecip && S != kZPI)) ? 3 : 1; }
 
 template <int S, int YF, bool HYB>
 __global__ __launch_bounds__(kYC* kYR, (y_min_waves<S, HYB>())) void k_flux_y(const double* __restrict__ U,
@@ -406,10 +413,12 @@ template <int XT>
 __host__ __device__ inline int x_tiles_x(const Geom& g) { return (g.nx + (XT - 1) - 1) / (XT - 1); }
 
 // The contracted WENO-ZPI x sweep would take 170 VGPRs and run at 2 waves per
-// SIMD; capped at 3 it fits without spilling (-10 % sweep time). The other
-// schemes fit by themselves (their code is left to the default heuristics).
+// SIMD; capped at 3 it fits without spilling (-10 % sweep time), as does the
+// switched JS / Z sweep with division by the reciprocal (246 otherwise). The
+// other sweeps fit by themselves (their code is left to the default
+// heuristics).
 template <int S, int XT, bool HYB>
-constexpr int x_min_waves() { return S == kZPI && !HYB ? 3 : 1; }
+constexpr int x_min_waves() { return (S == kZPI && !HYB) || (kCapRecip && HYB && S <= kZ) ? 3 : 1; }
 
 // With st, k.cdt holds the stage coefficient c and the step is c * st->dt.
 template <int S, int XT, bool HYB>

@@ -87,7 +87,10 @@ struct BatchView {
 //          without contraction);
 //   fused: the compiler contracts a * b + c into one FMA where it finds one
 //          (one rounding instead of two, as the reference's own optimised CPU
-//          build does with -march=native); fewer instructions.
+//          build does with -march=native) and divides by multiplying with the
+//          reciprocal (shared between divisions by one value, two roundings);
+//          fewer instructions. The WENO weights' explicit shared-reciprocal
+//          divisions stay exact in both builds.
 enum FpMode { kFpExact = 0, kFpFused = 1 };
 
 namespace exact {
