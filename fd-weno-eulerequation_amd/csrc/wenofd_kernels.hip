@@ -418,7 +418,7 @@ __host__ __device__ inline int x_tiles_x(const Geom& g) { return (g.nx + (XT - 1
 // other sweeps fit by themselves (their code is left to the default
 // heuristics).
 template <int S, int XT, bool HYB>
-constexpr int x_min_waves() { return (S == kZPI && !HYB) || (kCapRecip && HYB && S <= kZ) ? 3 : 1; }
+constexpr int x_min_waves() { return (S == kZPI && !HYB) || (kCapRecip && (!HYB || S <= kZ)) ? 3 : 1; }
 
 // With st, k.cdt holds the stage coefficient c and the step is c * st->dt.
 template <int S, int XT, bool HYB>

@@ -139,6 +139,18 @@ __device__ __forceinline__ double div_refined(double n, double d, double r) {
     return __builtin_fma(__builtin_fma(-d, q, n), r, q);
 }
 
+// n / d given r = rcp_refined(d): the correctly rounded quotient, or in the
+// build that divides by the reciprocal (WENOFD_RECIP_DIV) the product n r
+// (one rounding more, as every other division there).
+__device__ __forceinline__ double div_shared(double n, double d, double r) {
+#if WENOFD_RECIP_DIV
+    (void)d;
+    return n * r;
+#else
+    return div_refined(n, d, r);
+#endif
+}
+
 // weno5 with the weight normalisation (three divisions by the common sum s)
 // and WENO-Z's reciprocals 1/(beta + 1e-20) done by div_refined. ok is cleared
 // when an operand leaves the exact range: s >= 1 and 0.1 <= w_k <= s always
@@ -154,16 +166,16 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
         const double eps = 1.0e-6;
         const double d0 = (eps + b0) * (eps + b0), d1 = (eps + b1) * (eps + b1), d2 = (eps + b2) * (eps + b2);
         ok = ok && d0 <= kDivSafeMax && d1 <= kDivSafeMax && d2 <= kDivSafeMax;
-        w0 = div_refined(0.1, d0, rcp_refined(d0));
-        w1 = div_refined(0.6, d1, rcp_refined(d1));
-        w2 = div_refined(0.3, d2, rcp_refined(d2));
+        w0 = div_shared(0.1, d0, rcp_refined(d0));
+        w1 = div_shared(0.6, d1, rcp_refined(d1));
+        w2 = div_shared(0.3, d2, rcp_refined(d2));
     } else if constexpr (S == kZ) {
         const double eps = 1.0e-20;
         const double d0 = b0 + eps, d1 = b1 + eps, d2 = b2 + eps;
         ok = ok && d0 <= kDivSafeMax && d1 <= kDivSafeMax && d2 <= kDivSafeMax;
-        const double i0 = div_refined(1.0, d0, rcp_refined(d0));
-        const double i1 = div_refined(1.0, d1, rcp_refined(d1));
-        const double i2 = div_refined(1.0, d2, rcp_refined(d2));
+        const double i0 = div_shared(1.0, d0, rcp_refined(d0));
+        const double i1 = div_shared(1.0, d1, rcp_refined(d1));
+        const double i2 = div_shared(1.0, d2, rcp_refined(d2));
         const double tau = fabs(b0 - b2);
         w0 = 0.1 * (1.0 + (((tau * i0) * tau) * i0));
         w1 = 0.6 * (1.0 + (((tau * i1) * tau) * i1));
@@ -176,9 +188,9 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
     const double s = (w0 + w1) + w2;
     ok = ok && s <= kDivSafeMax;
     const double r = rcp_refined(s);
-    w0 = div_refined(w0, s, r);
-    w1 = div_refined(w1, s, r);
-    w2 = div_refined(w2, s, r);
+    w0 = div_shared(w0, s, r);
+    w1 = div_shared(w1, s, r);
+    w2 = div_shared(w2, s, r);
     weno_candidates(a, b, c, d, e, p0, p1, p2);
     return (w0 * p0 + w1 * p1) + w2 * p2;
 }
