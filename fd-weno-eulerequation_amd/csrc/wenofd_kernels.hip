@@ -31,6 +31,16 @@
 namespace wenofd {
 namespace WENOFD_FP_NS {
 
+// The build that divides by the reciprocal: its sweeps are capped at 3 waves
+// per SIMD where noted and recompute a face's IEEE fallback afresh (x sweeps,
+// and the WENO-ZPI y sweep: fewer registers there, more elsewhere and in the
+// build without contraction).
+#if WENOFD_RECIP_DIV
+constexpr bool kCapRecip = true;
+#else
+constexpr bool kCapRecip = false;
+#endif
+
 // ---------------------------------------------------------------------------
 // Ghost fill: exchangeGhostCellsValue (one-rank periodic-x ring) + setBoundary,
 // src/WENOFD.cpp:850-929 and :998-1159, restricted to the ghost points the
@@ -196,9 +206,9 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
                 Ur[r] = sU[r][(fr - base) % kRawRing][cx];
             }
             const int s0 = fr - 3 - base;  // ring position of point row fr-3
-            face_flux_dev<1, S>(
-                Ul, Ur, [&](int s, int r) { return sP[r][(s0 + s) % kRing][cx]; },
-                [&](int s, int r) { return sM[r][(s0 + s) % kRing][cx]; }, g.gamma, g.zp_lambda, Fh);
+            face_flux_dev<1, S, kCapRecip && S == kZPI>(
+                Ul, Ur, [&](int s, int r, int z) { return sP[r][(s0 + s) % kRing][cx + z]; },
+                [&](int s, int r, int z) { return sM[r][(s0 + s) % kRing][cx + z]; }, g.gamma, g.zp_lambda, Fh);
             const size_t o = (size_t)(fr - 4) * g.fy_pitch + (ic - 4);
 #pragma unroll
             for (int r = 0; r < 4; ++r) Fy[r * g.fy_plane + o] = Fh[r];
@@ -239,11 +249,6 @@ __host__ __device__ inline int y_tiles_y(const Geom& g) { return (g.ny + 1 + YF 
 // allow; capping it there (a 3-dword spill) runs 2 % faster than 2 waves
 // (for WENO-ZPI the 5-dword spill costs more than it gains). With division by
 // the reciprocal the JS and Z sweeps would take 246 VGPRs: capped likewise.
-#if WENOFD_RECIP_DIV
-constexpr bool kCapRecip = true;
-#else
-constexpr bool kCapRecip = false;
-#endif
 template <int S, bool HYB>
 constexpr int y_min_waves() { return !HYB && (S == kZP || (kCapRecip && S != kZPI)) ? 3 : 1; }
 
@@ -316,9 +321,9 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
             Ul[r] = sU[r][tt + 2];
             Ur[r] = sU[r][tt + 3];
         }
-        face_flux_dev<0, S>(
-            Ul, Ur, [&](int s, int r) { return sP[r][tt + s]; }, [&](int s, int r) { return sM[r][tt + s]; },
-            g.gamma, g.zp_lambda, Fh);
+        face_flux_dev<0, S, kCapRecip>(
+            Ul, Ur, [&](int s, int r, int z) { return sP[r][tt + s + z]; },
+            [&](int s, int r, int z) { return sM[r][tt + s + z]; }, g.gamma, g.zp_lambda, Fh);
 #pragma unroll
         for (int r = 0; r < 4; ++r) sF[r][tt] = Fh[r];
     };

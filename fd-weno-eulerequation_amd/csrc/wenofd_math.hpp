@@ -156,11 +156,91 @@ __device__ __forceinline__ double div_shared(double n, double d, double r) {
 // when an operand leaves the exact range: s >= 1 and 0.1 <= w_k <= s always
 // hold, so s <= 2^300 covers the normalisation; beta + 1e-20 >= 1e-20 always,
 // so an upper bound covers the reciprocals. NaN fails both checks.
+//
+// The build that divides by the reciprocal (WENOFD_RECIP_DIV) forms the JS, Z
+// and ZP weights over a common denominator instead: with B_k = (beta_k + eps),
+// D_k = B_k^2 and te = tau + eps, JS's d_k / D_k, Z's d_k (1 + tau^2 / D_k) and
+// ZP's d_k (1 + te^2 / D_k + lambda B_k / te) are proportional to d_k D_i D_j,
+// d_k (D_k + tau^2) D_i D_j and d_k (D_k (te + lambda B_k) + te^3) D_i D_j
+// ({i, j} the other two indices), and the normalisation cancels the common
+// factor. The weights are sums and products of positive terms (no
+// cancellation; a few roundings each), and the only reciprocal left is the one
+// of their sum s. Every term and partial product is >= 1e-282 (normal), so
+// s <= 2^1000 (which NaN and an overflowed term fail) keeps 1/s normal. ZPI's
+// weights are first scaled by 1/M (M = bmax + eps, one reciprocal more) and
+// stay within [0, 6.3] per factor; where a beta ratio underflows, s >= 2^-900
+// is required, which leaves every term that weighs more than 2^-60 of s normal.
 template <int S>
 __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c, double d, double e,
                                                    double zp_lambda, bool& ok) {
     double b0, b1, b2, w0, w1, w2, p0, p1, p2;
     weno_beta(a, b, c, d, e, b0, b1, b2);
+#if WENOFD_RECIP_DIV
+    {
+        // per scheme: the factors D_k (squared, shifted betas) and E_k
+        double D0, D1, D2, E0 = 1.0, E1 = 1.0, E2 = 1.0;
+        if constexpr (S == kZPI) {
+            // ZPI: d_k (1 + tau^2 / B_k^2 + 3.3 (1 - bmin / M) b_k / M) with
+            // M = bmax + eps, scaled by rM^2 B_k^2 (rM = 1/M): every ratio lies
+            // in [0, 1], so the scaled terms stay near 1 for any beta scale
+            const double eps = 1.0e-40;
+            const double tau = fabs(b0 - b2);
+            const double bmx12 = (b1 < b2) ? b2 : b1;
+            const double bmax = (b0 < bmx12) ? bmx12 : b0;
+            const double bmn12 = (b2 < b1) ? b2 : b1;
+            const double bmin = (bmn12 < b0) ? bmn12 : b0;
+            const double M = bmax + eps;
+            ok = ok && M <= 0x1p1000;
+            const double rM = rcp_refined(M);
+            const double x0 = (b0 + eps) * rM, x1 = (b1 + eps) * rM, x2 = (b2 + eps) * rM;
+            D0 = x0 * x0;
+            D1 = x1 * x1;
+            D2 = x2 * x2;
+            const double t = tau * rM;
+            const double t2 = t * t;
+            const double la = 3.3 * (1.0 - bmin * rM);
+            E0 = D0 * (1.0 + la * (b0 * rM)) + t2;
+            E1 = D1 * (1.0 + la * (b1 * rM)) + t2;
+            E2 = D2 * (1.0 + la * (b2 * rM)) + t2;
+        } else {
+            const double eps = S == kJS ? 1.0e-6 : S == kZ ? 1.0e-20 : 1.0e-40;
+            const double B0 = S == kJS ? eps + b0 : b0 + eps;
+            const double B1 = S == kJS ? eps + b1 : b1 + eps;
+            const double B2 = S == kJS ? eps + b2 : b2 + eps;
+            D0 = B0 * B0;
+            D1 = B1 * B1;
+            D2 = B2 * B2;
+            if constexpr (S == kZ) {
+                const double tau = fabs(b0 - b2);
+                const double t2 = tau * tau;
+                E0 = D0 + t2;
+                E1 = D1 + t2;
+                E2 = D2 + t2;
+            } else if constexpr (S == kZP) {
+                // 1 + te^2 / D_k + lambda B_k / te, times te: D_k (te + lambda B_k) + te^3
+                // (>= 1e-120; lambda = dx^(2/3) > 0 keeps every term positive)
+                const double te = fabs(b0 - b2) + eps;
+                const double te3 = (te * te) * te;
+                E0 = D0 * (te + zp_lambda * B0) + te3;
+                E1 = D1 * (te + zp_lambda * B1) + te3;
+                E2 = D2 * (te + zp_lambda * B2) + te3;
+                ok = ok && zp_lambda >= 0.0;
+            }
+        }
+        // d_k scaled by 10: (1, 6, 3)
+        w0 = E0 * (D1 * D2);
+        w1 = 6.0 * (E1 * (D0 * D2));
+        w2 = 3.0 * (E2 * (D0 * D1));
+        const double s = (w0 + w1) + w2;
+        ok = ok && s <= 0x1p1000 && (S != kZPI || s >= 0x1p-900);
+        const double r = rcp_refined(s);
+        w0 = w0 * r;
+        w1 = w1 * r;
+        w2 = w2 * r;
+        weno_candidates(a, b, c, d, e, p0, p1, p2);
+        return (w0 * p0 + w1 * p1) + w2 * p2;
+    }
+#endif
     if constexpr (S == kJS) {
         // d_k / (eps + beta_k)^2: denominators >= 1e-12
         const double eps = 1.0e-6;
@@ -351,7 +431,7 @@ WENOFD_HD void face_flux(const T Ul[4], const T Ur[4], GetP sp, GetM sm, double 
 // and placed after the fast path; the recomputation starts from the mean
 // state, so only that (not the face's two points) stays live across the fast
 // path.
-template <int D, int S, class GetP, class GetM>
+template <int D, int S, bool FRESH, class GetP, class GetM>
 __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double Ur[4], GetP sp, GetM sm, double gamma,
                                               double zp_lambda, double Fh[4]) {
     double ave[4], L[16], R[16];
@@ -364,8 +444,8 @@ __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double U
         double fp[4], fm[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            fp[r] = sp(s, r);
-            fm[r] = sm(s + 1, r);
+            fp[r] = sp(s, r, 0);
+            fm[r] = sm(s + 1, r, 0);
         }
         project_left<D>(L, fp, vp[s]);
         project_left<D>(L, fm, vm[s]);
@@ -380,8 +460,22 @@ __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double U
     }
     project_right<D>(R, vh, Fh);
     if (__any(!ok)) {
-        double Fe[4];
-        face_flux_ave<D, S>(ave, sp, sm, gamma, zp_lambda, Fe);
+        // with FRESH, recompute from the mean state and fresh LDS reads: the
+        // empty asm hides that they equal the fast path's, so none of its
+        // values (loaded points, projections, betas) is kept live across it for
+        // reuse here (fewer registers in some sweeps, more in others)
+        double av[4], Fe[4];
+        int k0 = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) av[r] = ave[r];
+        if constexpr (FRESH) {
+            asm volatile("" : "+v"(k0));
+#pragma unroll
+            for (int r = 0; r < 4; ++r) asm volatile("" : "+v"(av[r]));
+        }
+        face_flux_ave<D, S>(
+            av, [&](int s, int r) { return sp(s, r, k0); }, [&](int s, int r) { return sm(s, r, k0); }, gamma,
+            zp_lambda, Fe);
         if (!ok) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) Fh[r] = Fe[r];
