@@ -164,7 +164,9 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
     __shared__ int sIdx[HYB ? kYC * kYR : 1];
     __shared__ int sCnt[kYR];
     const int tx = threadIdx.x % kYC;
-    const int ty = threadIdx.x / kYC;
+    // one wave = one face row of a step: ty (and every ring row index derived
+    // from it) is wave-uniform, which readfirstlane makes scalar arithmetic
+    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x / kYC);
     const int i = 4 + bx * kYC + tx;
     const bool col_ok = i < g.nx + 4;
     const int F0 = 4 + by * YF;             // first face of the tile
