@@ -282,6 +282,8 @@ int create_impl(const char* const* texts, int n, int device, wenofd_batch** out)
         g.fy_plane = (long)g.fy_pitch * (g.ny + 1);
         g.dx = b->grid[s].dx;
         g.dy = b->grid[s].dy;
+        g.rdx = 1.0 / g.dx;
+        g.rdy = 1.0 / g.dy;
         g.gamma = b->cs[s].gamma;
         g.zp_lambda = b->grid[s].zp_lambda;
         d.bc = Bc{b->cs[s].bc_left, b->cs[s].bc_right, b->cs[s].bc_bottom};

@@ -370,6 +370,8 @@ int create_impl(const std::string& text, int device, wenofd_solver** out) {
     g.fy_plane = (long)g.fy_pitch * (g.ny + 1);
     g.dx = h->grid.dx;
     g.dy = h->grid.dy;
+    g.rdx = 1.0 / g.dx;
+    g.rdy = 1.0 / g.dy;
     g.gamma = h->cs.gamma;
     g.zp_lambda = h->grid.zp_lambda;
     h->bc = Bc{h->cs.bc_left, h->cs.bc_right, h->cs.bc_bottom};

@@ -263,6 +263,26 @@ __global__ __launch_bounds__(kYC* kYR, (y_min_waves<S, HYB>())) void k_flux_y(co
     flux_y_tile<S, YF, HYB>(U, Fy, g, blockIdx.x, blockIdx.y, flags);
 }
 
+// The cell terms with the spacings. The build that divides by the reciprocal
+// multiplies by the host's 1/dx and 1/dy, the reciprocals its own divisions
+// would otherwise compute in every thread.
+__device__ __forceinline__ double cell_rhs_g(double fxl, double fxr, double fyb, double fyt, const Geom& g) {
+#if WENOFD_RECIP_DIV
+    return -(fxr - fxl) * g.rdx + -(fyt - fyb) * g.rdy;
+#else
+    return cell_rhs(fxl, fxr, fyb, fyt, g.dx, g.dy);
+#endif
+}
+
+__device__ __forceinline__ double local_dt_g(const double U[4], const Geom& g, double cfl) {
+#if WENOFD_RECIP_DIV
+    const Prim<double> q = primitive(U[0], U[1], U[2], U[3], g.gamma);
+    return cfl / ((fabs(q.u) + q.c) * g.rdx + (fabs(q.v) + q.c) * g.rdy);
+#else
+    return local_dt(U, g.gamma, cfl, g.dx, g.dy);
+#endif
+}
+
 // ---------------------------------------------------------------------------
 // x faces fused with the rhs and the RK stage update.
 //
@@ -368,7 +388,7 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
         double rhs[4], u[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            rhs[r] = cell_rhs(sF[r][t], sF[r][t + 1], fyb[r], fyt[r], g.dx, g.dy);
+            rhs[r] = cell_rhs_g(sF[r][t], sF[r][t + 1], fyb[r], fyt[r], g);
             u[r] = sU[r][t + 3];
         }
         if (k.gravity) {  // src/WENOFD.cpp:812-813
@@ -387,7 +407,7 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
 #pragma unroll
             for (int r = 0; r < 4; ++r) Uout[r * g.plane + o] = nu[r];
             if (part) {
-                const double tl = local_dt(nu, g.gamma, cfl, g.dx, g.dy);
+                const double tl = local_dt_g(nu, g, cfl);
                 if (tl != tl) my_nan = 1;
                 my_min = (tl < my_min) ? tl : my_min;
             }
@@ -479,7 +499,7 @@ __device__ __forceinline__ void dt_point(const double* __restrict__ U, const Geo
     const int jl = (int)(k / g.nx), il = (int)(k % g.nx);
     const size_t o = (size_t)(jl + 4) * g.pitch + (il + 4);
     const double u[4] = {U[o], U[g.plane + o], U[2 * g.plane + o], U[3 * g.plane + o]};
-    const double tmp = local_dt(u, g.gamma, cfl, g.dx, g.dy);
+    const double tmp = local_dt_g(u, g, cfl);
     if (tmp != tmp) nan = 1;
     m = (tmp < m) ? tmp : m;
 }

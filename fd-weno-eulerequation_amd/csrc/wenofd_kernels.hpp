@@ -15,6 +15,7 @@ struct Geom {
     int fy_pitch;      // doubles per row of a y-face plane (>= nx), ny + 1 rows
     long fy_plane;
     double dx, dy;     // point spacing (xR - xL) / (nx - 1), likewise y
+    double rdx, rdy;   // 1 / dx, 1 / dy (IEEE division on the host)
     double gamma;
     double zp_lambda;  // pow(dx, 2/3), used by WENO-ZP in both sweeps
 };
