@@ -258,9 +258,20 @@ int run_rk_switched(wenofd_solver* h, int rkmeth, double dt) {
     return 0;
 }
 
+// The ghost points an RK stage's x sweep writes into its output (RK2 / RK3):
+// the next stage then needs no ghost fill. Off for grids under 5 points a side.
+GhostOut stage_ghosts(const wenofd_solver* h) {
+    GhostOut gh;
+    gh.on = h->g.nx >= 5 && h->g.ny >= 5;
+    gh.bc = h->bc;
+    gh.special = h->special;
+    return gh;
+}
+
 int run_rk(wenofd_solver* h, int rkmeth, double dt) {
     if (h->sw_mode != WENOFD_SWITCH_OFF) return run_rk_switched(h, rkmeth, dt);
     const int grav = h->opt.testcase == kRTI ? 1 : 0;
+    const GhostOut gh = stage_ghosts(h);
     int rc;
     h->dt_parts = 0;
     h->prev_slot = nullptr;
@@ -271,18 +282,18 @@ int run_rk(wenofd_solver* h, int rkmeth, double dt) {
         return 0;
     }
     if (rkmeth == 1) {  // RunRK2, :312-351 (Un = U)
-        RkCoef k1{kRkStage, grav, 1.0, 0.0, 1.0 * dt};
-        RkCoef k2{kRkStage, grav, 0.5, 0.5, 0.5 * dt};
+        RkCoef k1{kRkStage, grav, 1.0, 0.0, 1.0 * dt, gh};
+        RkCoef k2{kRkStage, grav, 0.5, 0.5, 0.5 * dt, gh};
         if ((rc = stage(h, h->U, h->U, h->Ua, k1, nullptr, false))) return rc;
-        return stage(h, h->Ua, h->U, h->U, k2, nullptr, true);
+        return stage(h, h->Ua, h->U, h->U, k2, nullptr, true, !gh.on);
     }
     if (rkmeth == 2) {  // RunRK3, :352-396 (Un = U)
-        RkCoef k1{kRkStage, grav, 1.0, 0.0, 1.0 * dt};
-        RkCoef k2{kRkStage, grav, 3.0 / 4.0, 1.0 / 4.0, (1.0 / 4.0) * dt};
-        RkCoef k3{kRkStage, grav, 1.0 / 3.0, 2.0 / 3.0, (2.0 / 3.0) * dt};
+        RkCoef k1{kRkStage, grav, 1.0, 0.0, 1.0 * dt, gh};
+        RkCoef k2{kRkStage, grav, 3.0 / 4.0, 1.0 / 4.0, (1.0 / 4.0) * dt, gh};
+        RkCoef k3{kRkStage, grav, 1.0 / 3.0, 2.0 / 3.0, (2.0 / 3.0) * dt, gh};
         if ((rc = stage(h, h->U, h->U, h->Ua, k1, nullptr, false))) return rc;
-        if ((rc = stage(h, h->Ua, h->U, h->Ub, k2, nullptr, false))) return rc;
-        return stage(h, h->Ub, h->U, h->U, k3, nullptr, true);
+        if ((rc = stage(h, h->Ua, h->U, h->Ub, k2, nullptr, false, !gh.on))) return rc;
+        return stage(h, h->Ub, h->U, h->U, k3, nullptr, true, !gh.on);
     }
     return fail(WENOFD_E_BAD_ENUM, "RKMETH must be 0, 1 or 2");
 }
@@ -489,15 +500,20 @@ int loop_setup(wenofd_solver* h) {
 // the loop test |now - t_end| > 1e-9), the stages run the plain kernels with
 // the step read from device memory (nothing runs once the solve is done), and
 // k_batch_advance moves the clock. Same kernels and coefficients as run_rk.
-int enqueue_device_step(wenofd_solver* h, const BatchView& v) {
+// With RK2 / RK3 the stages' x sweeps leave their outputs ghost-filled, so
+// only the first step of a launch sequence (first) fills the ghosts of U.
+int enqueue_device_step(wenofd_solver* h, const BatchView& v, bool first) {
     const SolveState* st = h->loop_st_d;
     const int grav = h->opt.testcase == kRTI ? 1 : 0;
+    const GhostOut gh = stage_ghosts(h);
     auto chk = [](hipError_t e) {
         return e == hipSuccess ? 0 : fail(WENOFD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     };
-    auto stage_dev = [&](double* in, const double* un, double* out, const RkCoef& k, bool last) -> int {
+    auto stage_dev = [&](double* in, const double* un, double* out, RkCoef k, bool last, bool fill) -> int {
         int rc;
-        if ((rc = chk(launch_ghost_fill(in, h->special, h->g, h->bc, h->stream, st)))) return rc;
+        if (h->opt.rkmeth != 0) k.gh = gh;
+        if ((fill || !k.gh.on) && (rc = chk(launch_ghost_fill(in, h->special, h->g, h->bc, h->stream, st))))
+            return rc;
         const int fp = h->opt.fp_contract;
         if ((rc = chk(WENOFD_FP(fp, launch_flux_y(h->opt.scheme, h->tiles, in, h->Fy, h->g, h->stream, nullptr, st)))))
             return rc;
@@ -509,16 +525,18 @@ int enqueue_device_step(wenofd_solver* h, const BatchView& v) {
     if ((rc = chk(launch_batch_dt(v, h->stream)))) return rc;
     // RkCoef.cdt carries the stage coefficient c; the kernels multiply by dt
     if (h->opt.rkmeth == 0) {
-        if ((rc = stage_dev(h->U, nullptr, h->Ua, RkCoef{kRk1, grav, 0.0, 1.0, 1.0}, true))) return rc;
+        if ((rc = stage_dev(h->U, nullptr, h->Ua, RkCoef{kRk1, grav, 0.0, 1.0, 1.0}, true, true))) return rc;
         if ((rc = chk(launch_copy_interior(h->Ua, h->U, h->g, st, h->stream)))) return rc;
     } else if (h->opt.rkmeth == 1) {
-        if ((rc = stage_dev(h->U, h->U, h->Ua, RkCoef{kRkStage, grav, 1.0, 0.0, 1.0}, false))) return rc;
-        if ((rc = stage_dev(h->Ua, h->U, h->U, RkCoef{kRkStage, grav, 0.5, 0.5, 0.5}, true))) return rc;
+        if ((rc = stage_dev(h->U, h->U, h->Ua, RkCoef{kRkStage, grav, 1.0, 0.0, 1.0}, false, first))) return rc;
+        if ((rc = stage_dev(h->Ua, h->U, h->U, RkCoef{kRkStage, grav, 0.5, 0.5, 0.5}, true, false))) return rc;
     } else {
-        if ((rc = stage_dev(h->U, h->U, h->Ua, RkCoef{kRkStage, grav, 1.0, 0.0, 1.0}, false))) return rc;
-        if ((rc = stage_dev(h->Ua, h->U, h->Ub, RkCoef{kRkStage, grav, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0}, false)))
+        if ((rc = stage_dev(h->U, h->U, h->Ua, RkCoef{kRkStage, grav, 1.0, 0.0, 1.0}, false, first))) return rc;
+        if ((rc = stage_dev(h->Ua, h->U, h->Ub, RkCoef{kRkStage, grav, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0}, false,
+                            false)))
             return rc;
-        if ((rc = stage_dev(h->Ub, h->U, h->U, RkCoef{kRkStage, grav, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}, true)))
+        if ((rc = stage_dev(h->Ub, h->U, h->U, RkCoef{kRkStage, grav, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}, true,
+                            false)))
             return rc;
     }
     return chk(launch_batch_advance(v, false, h->stream));
@@ -533,7 +551,7 @@ int enqueue_device_steps(wenofd_solver* h, const BatchView& v, int n) {
         if (!L.exec) {
             HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
             rc = 0;
-            for (int k = 0; k < L.graph_steps && !rc; ++k) rc = enqueue_device_step(h, v);
+            for (int k = 0; k < L.graph_steps && !rc; ++k) rc = enqueue_device_step(h, v, k == 0);
             hipGraph_t graph = nullptr;
             const hipError_t e = hipStreamEndCapture(h->stream, &graph);
             if (rc) {
@@ -553,8 +571,8 @@ int enqueue_device_steps(wenofd_solver* h, const BatchView& v, int n) {
         }
         for (; n >= L.graph_steps; n -= L.graph_steps) HIP_TRY(hipGraphLaunch(L.exec, h->stream));
     }
-    for (; n > 0; --n)
-        if ((rc = enqueue_device_step(h, v))) return rc;
+    for (bool first = true; n > 0; --n, first = false)
+        if ((rc = enqueue_device_step(h, v, first))) return rc;
     return 0;
 }
 

@@ -107,6 +107,76 @@ __device__ __forceinline__ void ghost_point(double* __restrict__ U, const double
     }
 }
 
+// ghost_point inverted: the ghost points whose source is interior point (i, j)
+// of U, holding v, get their values from it (copies, or the normal momentum
+// flipped at a slip side); a SPECIAL side's four ghosts next to (i, j) take
+// their fixed values. With nx, ny >= 5 every ghost of the fill has an interior
+// source, so calling this for every interior point leaves U ghost-filled
+// exactly as k_ghost_fill would. xs / ys (uniform over the caller's block)
+// enable the x-ghost part (points within 5 of the left or right edge) and the
+// y-ghost part (within 5 of the bottom or top).
+__device__ __forceinline__ void ghost_from_point(double* __restrict__ U, const Geom& g, const GhostOut& gh, int i,
+                                                 int j, const double v[4], bool xs, bool ys) {
+    const int sx = 4, ex = g.nx + 4, sy = 4, ey = g.ny + 4;
+    auto put = [&](int di, int dj, int flip) {
+        const size_t d = (size_t)dj * g.pitch + di;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) U[r * g.plane + d] = (r == flip) ? -v[r] : v[r];
+    };
+    auto put_special = [&](int di, int dj, int spec) {
+        const size_t d = (size_t)dj * g.pitch + di;
+        const int ns = 2 * (g.nx + g.ny);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) U[r * g.plane + d] = gh.special[r * ns + spec];
+    };
+    // left ghosts i = sx - e1 and right ghosts i = ex - 1 + e1 of row j
+    if (!xs) {
+    } else if (gh.bc.left == kBcPeriod) {
+        const int e1 = ex - 1 - i;
+        if (e1 >= 1 && e1 <= 4) put(sx - e1, j, -1);
+    } else if (gh.bc.left == kBcSpecial) {
+        if (i == sx)
+#pragma unroll 1
+            for (int e1 = 1; e1 <= 4; ++e1) put_special(sx - e1, j, 2 * g.nx + (j - sy));
+    } else {
+        const int e1 = i - sx;
+        if (e1 >= 1 && e1 <= 4) put(sx - e1, j, gh.bc.left == kBcSlip ? 1 : -1);
+    }
+    if (!xs) {
+    } else if (gh.bc.right == kBcPeriod) {
+        const int e1 = i - sx;
+        if (e1 >= 1 && e1 <= 4) put(ex - 1 + e1, j, -1);
+    } else if (gh.bc.right == kBcSpecial) {
+        if (i == ex - 1)
+#pragma unroll 1
+            for (int e1 = 1; e1 <= 4; ++e1) put_special(ex - 1 + e1, j, 2 * g.nx + g.ny + (j - sy));
+    } else {
+        const int e1 = ex - 1 - i;
+        if (e1 >= 1 && e1 <= 4) put(ex - 1 + e1, j, gh.bc.right == kBcSlip ? 1 : -1);
+    }
+    // bottom ghosts j = sy - e1 and top ghosts j = ey - 1 + e1 of column i
+    if (!ys) {
+    } else if (gh.bc.bottom == kBcPeriod) {
+        const int e1 = ey - 1 - j;
+        if (e1 >= 1 && e1 <= 4) put(i, sy - e1, -1);
+        const int e2 = j - sy;
+        if (e2 >= 1 && e2 <= 4) put(i, ey - 1 + e2, -1);
+    } else if (gh.bc.bottom == kBcSpecial) {
+        if (j == sy)
+#pragma unroll 1
+            for (int e1 = 1; e1 <= 4; ++e1) put_special(i, sy - e1, i - sx);
+        if (j == ey - 1)
+#pragma unroll 1
+            for (int e1 = 1; e1 <= 4; ++e1) put_special(i, ey - 1 + e1, g.nx + (i - sx));
+    } else {
+        const int flip = gh.bc.bottom == kBcSlip ? 2 : -1;
+        const int e1 = j - sy;
+        if (e1 >= 1 && e1 <= 4) put(i, sy - e1, flip);
+        const int e2 = ey - 1 - j;
+        if (e2 >= 1 && e2 <= 4) put(i, ey - 1 + e2, flip);
+    }
+}
+
 // st (optional): the solve's device-side time state; an inactive solve skips
 // the launch (device loop, see wenofd_devloop.hpp).
 __global__ __launch_bounds__(kGhostThreads) void k_ghost_fill(double* __restrict__ U,
@@ -370,6 +440,11 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
     }
     const int i = f0 + t;  // this thread's cell
     const bool cell_ok = t < XC && i < g.nx + 4;
+    // ghost points of Uout to write: only the waves whose cells reach within 5
+    // of the x edges, and rows within 5 of the y edges, have any (wave-uniform)
+    const int wf = __builtin_amdgcn_readfirstlane(f0 + (t & ~63));  // the wave's first cell
+    const bool gh_x = k.gh.on && (wf <= 4 + 4 || wf + 63 >= g.nx + 4 - 5);
+    const bool gh_y = k.gh.on && (j <= 4 + 4 || j >= g.ny + 4 - 5);
     const size_t o = (size_t)j * g.pitch + i;
     const size_t oy = (size_t)(j - 4) * g.fy_pitch + (i - 4);
     double fyb[4], fyt[4], unb[4];
@@ -406,6 +481,7 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
                 nu[r] = (k.mode == kRk1) ? u[r] + k.cdt * rhs[r] : rk_combine(k.a, unb[r], k.b, u[r], k.cdt, rhs[r]);
 #pragma unroll
             for (int r = 0; r < 4; ++r) Uout[r * g.plane + o] = nu[r];
+            if (gh_x || gh_y) ghost_from_point(Uout, g, k.gh, i, j, nu, gh_x, gh_y);
             if (part) {
                 const double tl = local_dt_g(nu, g, cfl);
                 if (tl != tl) my_nan = 1;

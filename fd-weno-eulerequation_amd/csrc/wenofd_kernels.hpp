@@ -26,10 +26,22 @@ struct Bc {
 
 enum RkMode { kRkNone = -1, kRk1 = 0, kRkStage = 1 };
 
+// With on set, an x sweep that updates cells also writes the ghost points of
+// its output that the next stage's ghost fill would derive from them (bc, and
+// the SPECIAL values for the sides of that type): the output leaves the sweep
+// ghost-filled. Valid for nx, ny >= 5, where every ghost copies an interior
+// point.
+struct GhostOut {
+    int on = 0;
+    Bc bc{};
+    const double* special = nullptr;
+};
+
 struct RkCoef {
     int mode;
     int gravity;       // RTI source term (g = 1)
     double a, b, cdt;  // U = (a Un + b U) + cdt rhs; cdt = c * dt
+    GhostOut gh{};
 };
 
 // Tile shapes: faces per block along x (the fused x kernel: 64, 128 or 256)
