@@ -356,11 +356,11 @@ __device__ __forceinline__ double local_dt_g(const double U[4], const Geom& g, d
 // ---------------------------------------------------------------------------
 // x faces fused with the rhs and the RK stage update.
 //
-// One tile = XT consecutive faces of one interior row = the XT-1 cells between
-// them (consecutive tiles overlap by one face). The LLF split of the XT+5
-// points the faces read is computed once into LDS (component-major: a wave's
-// 64 consecutive slots hit 64 distinct banks), then each thread builds one face
-// flux into LDS and completes one cell:
+// One tile = XT-5 consecutive faces of one interior row = the XT-6 cells
+// between them (consecutive tiles overlap by one face). The LLF split of the
+// XT points the faces read is computed once into LDS, one point per thread
+// (component-major: a wave's 64 consecutive slots hit 64 distinct banks), then
+// each thread builds one face flux into LDS and completes one cell:
 //   rhs = x term + y term (faces read from Fy) [+ RTI source], then
 //   kRkNone:  rhs only (rhs_out)           kRk1:   out = in + dt rhs
 //   kRkStage: out = (a Un + b in) + (c dt) rhs
@@ -369,14 +369,19 @@ __device__ __forceinline__ double local_dt_g(const double U[4], const Geom& g, d
 // tile also reduces the local CFL step of its updated cells (the next
 // calculateDeltaT) into *part / *part_nan.
 // ---------------------------------------------------------------------------
+// Cells per x tile of XT threads: XT - 5 faces read exactly XT points, so
+// every thread splits one point.
+template <int XT>
+constexpr int x_cells() { return XT - 6; }
+
 template <int S, int XT, bool HYB = false>
 __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, const double* __restrict__ Fy,
                                                const double* Un, double* Uout, double* __restrict__ rhs_out,
                                                const Geom& g, const RkCoef& k, double cfl, double* part,
                                                int* part_nan, int bx, int row,
                                                const unsigned char* __restrict__ flags = nullptr) {
-    constexpr int XC = XT - 1;  // cells per tile
-    constexpr int XP = XT + 5;  // points per tile
+    constexpr int XC = x_cells<XT>();  // cells per tile
+    constexpr int XP = XC + 6;         // points per tile (= XT)
     constexpr int XS = XT + 8;  // LDS row stride (slots)
     constexpr int W = XT / 64;  // waves per tile
     __shared__ double sP[4][XS];
@@ -423,7 +428,7 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
         __shared__ int sIdx[XT];
         __shared__ int sCnt[W];
         bool fl = false;
-        if (f <= g.nx + 4) {
+        if (t <= XC && f <= g.nx + 4) {
             fl = flags[(size_t)row * (g.nx + 1) + (f - 4)] != 0;
             if (!fl) {
                 double Fh[4];
@@ -436,7 +441,7 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
         const int nfl = compact_flagged<XT>(fl, sIdx, sCnt);
         for (int q = t; q < nfl; q += XT) full_face(sIdx[q]);
     } else {
-        if (f <= g.nx + 4) full_face(t);
+        if (t <= XC && f <= g.nx + 4) full_face(t);
     }
     const int i = f0 + t;  // this thread's cell
     const bool cell_ok = t < XC && i < g.nx + 4;
@@ -513,7 +518,7 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
 }
 
 template <int XT>
-__host__ __device__ inline int x_tiles_x(const Geom& g) { return (g.nx + (XT - 1) - 1) / (XT - 1); }
+__host__ __device__ inline int x_tiles_x(const Geom& g) { return (g.nx + x_cells<XT>() - 1) / x_cells<XT>(); }
 
 // The contracted WENO-ZPI x sweep would take 170 VGPRs and run at 2 waves per
 // SIMD; capped at 3 it fits without spilling (-10 % sweep time), as does the
@@ -772,9 +777,9 @@ __global__ __launch_bounds__(XT) void k_batch_copy(BatchView b, int from, int to
     SolveDesc d;
     SolveState st;
     if (!batch_block(b, tiles, tr, d, st)) return;
-    const int i = 4 + tr.bx * (XT - 1) + threadIdx.x;
+    const int i = 4 + tr.bx * x_cells<XT>() + threadIdx.x;
     const int j = 4 + tr.by;
-    if (threadIdx.x >= XT - 1 || i >= d.g.nx + 4) return;
+    if (threadIdx.x >= x_cells<XT>() || i >= d.g.nx + 4) return;
     const size_t o = d.u_off + (size_t)j * d.g.pitch + i;
 #pragma unroll
     for (int r = 0; r < 4; ++r) b.U[to][o + r * d.g.plane] = b.U[from][o + r * d.g.plane];
