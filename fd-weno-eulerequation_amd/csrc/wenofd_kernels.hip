@@ -319,10 +319,12 @@ __host__ __device__ inline int y_tiles_y(const Geom& g) { return (g.ny + 1 + YF 
 
 // The WENO-ZP y sweep wants a few more registers than 3 waves per SIMD
 // allow; capping it there (a 3-dword spill) runs 2 % faster than 2 waves
-// (for WENO-ZPI the 5-dword spill costs more than it gains). With division by
-// the reciprocal the JS and Z sweeps would take 246 VGPRs: capped likewise.
+// (for WENO-ZPI the 5-dword spill costs more than it gains). The build that
+// divides by the reciprocal caps every scheme: JS and Z would otherwise take
+// 246 VGPRs, and its ZPI sweep runs 4 % faster capped (a 9-dword spill) than
+// at 2 waves.
 template <int S, bool HYB>
-constexpr int y_min_waves() { return !HYB && (S == kZP || (kCapRecip && S != kZPI)) ? 3 : 1; }
+constexpr int y_min_waves() { return !HYB && (S == kZP || kCapRecip) ? 3 : 1; }
 
 template <int S, int YF, bool HYB>
 __global__ __launch_bounds__(kYC* kYR, (y_min_waves<S, HYB>())) void k_flux_y(const double* __restrict__ U,
