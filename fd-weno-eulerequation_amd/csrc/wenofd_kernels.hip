@@ -335,6 +335,31 @@ __global__ __launch_bounds__(kYC* kYR, (y_min_waves<S, HYB>())) void k_flux_y(co
     flux_y_tile<S, YF, HYB>(U, Fy, g, blockIdx.x, blockIdx.y, flags);
 }
 
+// Minimum over the wave's 64 lanes of values that are never NaN, returned to
+// every lane: DPP lane exchanges (quad permutations, row mirrors, then the
+// row broadcasts of lanes 15 and 31) leave the minimum in lane 63. No LDS
+// round trips; MIN is exact, so the order of the exchanges does not matter.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = (int)b, hi = (int)(b >> 32);
+    const int l2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWS, 0xF, false);
+    const int h2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWS, 0xF, false);
+    return __longlong_as_double(((long long)h2 << 32) | (unsigned int)l2);
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+    v = fmin(v, dpp_f64<0xB1>(v));        // quad_perm [1, 0, 3, 2]
+    v = fmin(v, dpp_f64<0x4E>(v));        // quad_perm [2, 3, 0, 1]
+    v = fmin(v, dpp_f64<0x141>(v));       // row_half_mirror
+    v = fmin(v, dpp_f64<0x140>(v));       // row_mirror
+    v = fmin(v, dpp_f64<0x142, 0xA>(v));  // row_bcast:15 into rows 1 and 3
+    v = fmin(v, dpp_f64<0x143, 0xC>(v));  // row_bcast:31 into rows 2 and 3
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // The cell terms with the spacings. The build that divides by the reciprocal
 // multiplies by the host's 1/dx and 1/dy, the reciprocals its own divisions
 // would otherwise compute in every thread.
@@ -497,12 +522,8 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
         }
     }
     if (part) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(my_min, off, 64);
-            my_min = (ov < my_min) ? ov : my_min;
-            my_nan |= __shfl_xor(my_nan, off, 64);
-        }
+        my_min = wave_min(my_min);
+        my_nan = __any(my_nan) ? 1 : 0;
         if (t % 64 == 0) {
             sMin[t / 64] = my_min;
             sNan[t / 64] = my_nan;
@@ -556,13 +577,8 @@ __global__ __launch_bounds__(XT, (x_min_waves<S, XT, HYB>())) void k_flux_x_rk(c
 // ---------------------------------------------------------------------------
 template <int NW>
 __device__ __forceinline__ void min_nan_reduce_block(double& m, int& nan, double* sm, int* sn) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_xor(m, off, 64);
-        const int on = __shfl_xor(nan, off, 64);
-        m = (o < m) ? o : m;
-        nan |= on;
-    }
+    m = wave_min(m);
+    nan = __any(nan) ? 1 : 0;
     const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
     if (lane == 0) {
         sm[w] = m;
