@@ -12,6 +12,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace wenofd {
 
 // Every function is a template on the scalar type T: the kernels instantiate
@@ -275,16 +277,42 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
     return (w0 * p0 + w1 * p1) + w2 * p2;
 }
 
+// Device code of the build that divides by the reciprocal: divisions by the
+// density and the sound speed multiply by rcp_refined (v_rcp_f64 and two
+// Newton steps, the correctly rounded reciprocal on every sampled normal
+// operand) instead of the compiler's reciprocal, a full IEEE 1/x. A zero,
+// infinite or NaN density or sound speed (non-physical states) gives NaN
+// where the division gives inf or NaN.
+#if WENOFD_RECIP_DIV && defined(__HIP_DEVICE_COMPILE__)
+#define WENOFD_RCP_STATE 1
+#else
+#define WENOFD_RCP_STATE 0
+#endif
+
 // Primitive variables with the reference's expression forms
 // (utils/equations.hpp:139-144): p = (g-1)(E - (0.5 rho)(u^2+v^2)), c = sqrt((g p)/rho).
+// irho = 1/rho is set only where WENOFD_RCP_STATE computes it.
 template <class T>
 struct Prim {
-    T rho, u, v, E, p, c;
+    T rho, u, v, E, p, c, irho;
 };
 
 template <class T>
 WENOFD_HD Prim<T> primitive(T U0, T U1, T U2, T U3, double gamma) {
     Prim<T> q;
+#if WENOFD_RCP_STATE
+    if constexpr (std::is_same<T, double>::value) {
+        const double r = rcp_refined(U0);
+        q.rho = U0;
+        q.irho = r;
+        q.u = U1 * r;
+        q.v = U2 * r;
+        q.E = U3;
+        q.p = (gamma - 1.0) * (q.E - (0.5 * q.rho) * (q.u * q.u + q.v * q.v));
+        q.c = sqrt((gamma * q.p) * r);
+        return q;
+    }
+#endif
     q.rho = U0;
     q.u = U1 / U0;
     q.v = U2 / U0;
@@ -328,14 +356,24 @@ WENOFD_HD void flux_split(const T U[4], double gamma, T Fm[4], T Fp[4]) {
 template <int D, class T>
 WENOFD_HD void eigen_matrices(const T U[4], double gamma, T L[16], T R[16]) {
     const Prim<T> q = primitive(U[0], U[1], U[2], U[3], gamma);
+#if WENOFD_RCP_STATE
+    const T B1 = (gamma - 1.0) * rcp_refined(q.c * q.c);
+    const T ic = rcp_refined(q.c);
+    const T H = (q.E + q.p) * q.irho;
+#else
     const T B1 = (gamma - 1.0) / (q.c * q.c);
-    const T uu = q.u * q.u + q.v * q.v;
-    const T B2 = (B1 * uu) / 2.0;
     const T ic = 1.0 / q.c;
     const T H = (q.E + q.p) / q.rho;
+#endif
+    const T uu = q.u * q.u + q.v * q.v;
+    const T B2 = (B1 * uu) / 2.0;
     const T B1u = B1 * q.u, B1v = B1 * q.v;
     if constexpr (D == 0) {
+#if WENOFD_RCP_STATE
+        const T uc = q.u * ic;
+#else
         const T uc = q.u / q.c;
+#endif
         L[0] = (B2 + uc) / 2.0;  L[1] = -(B1u + ic) / 2.0;  L[2] = -B1v / 2.0;  L[3] = B1 / 2.0;
         L[4] = q.v;              L[5] = 0.0;                L[6] = -1.0;        L[7] = 0.0;
         L[8] = 1.0 - B2;         L[9] = B1u;                L[10] = B1v;        L[11] = -B1;
@@ -345,7 +383,11 @@ WENOFD_HD void eigen_matrices(const T U[4], double gamma, T L[16], T R[16]) {
         R[8] = q.v;              R[9] = -1.0;  R[10] = q.v;       R[11] = q.v;
         R[12] = H - q.c * q.u;   R[13] = -q.v; R[14] = uu / 2.0;  R[15] = H + q.c * q.u;
     } else {
+#if WENOFD_RCP_STATE
+        const T vc = q.v * ic;
+#else
         const T vc = q.v / q.c;
+#endif
         L[0] = (B2 + vc) / 2.0;  L[1] = -B1u / 2.0; L[2] = -(B1v + ic) / 2.0;  L[3] = B1 / 2.0;
         L[4] = -q.u;             L[5] = 1.0;        L[6] = 0.0;                L[7] = 0.0;
         L[8] = 1.0 - B2;         L[9] = B1u;        L[10] = B1v;               L[11] = -B1;
