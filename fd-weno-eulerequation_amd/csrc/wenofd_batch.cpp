@@ -80,34 +80,39 @@ int check_launch(hipError_t e) {
     return e == hipSuccess ? 0 : fail(WENOFD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
 }
 
-// One stage of every active solve: ghost fill of `in`, y faces, x faces fused
-// with the rhs and the update into `out` (Un = buffer 0).
+// One stage of every active solve: ghost fill of `in` (with fill), y faces, x
+// faces fused with the rhs and the update into `out` (Un = buffer 0), which
+// with ghosts also writes the ghost points of `out`.
 int stage(DevLoop& L, const BatchView& v, int in, int out, int mode, double a, double bc, double c, bool last,
-          hipStream_t s) {
+          bool fill, bool ghosts, hipStream_t s) {
     int rc;
-    if ((rc = check_launch(launch_batch_ghost(v, in, L.tab_ghost, L.n_ghost, s)))) return rc;
+    if (fill && (rc = check_launch(launch_batch_ghost(v, in, L.tab_ghost, L.n_ghost, s)))) return rc;
     if ((rc = check_launch(WENOFD_FP(L.fp, launch_batch_flux_y(L.scheme, L.tiles, v, in, L.tab_y, L.n_y, s)))))
         return rc;
-    return check_launch(
-        WENOFD_FP(L.fp, launch_batch_flux_x_rk(L.scheme, L.tiles, v, in, out, mode, a, bc, c, last, L.tab_x, L.n_x, s)));
+    return check_launch(WENOFD_FP(
+        L.fp, launch_batch_flux_x_rk(L.scheme, L.tiles, v, in, out, mode, a, bc, c, last, ghosts, L.tab_x, L.n_x, s)));
 }
 
 // One iteration of the run() loop for every solve (src/WENOFD.cpp:213-239):
 // calculateDeltaT, RunRK<RKMETH>, now += dt. The RK coefficients are those of
-// src/WENOFD.cpp:301-396 (cdt = c * dt formed in the kernel).
-int enqueue_step(DevLoop& L, const BatchView& v, hipStream_t s) {
+// src/WENOFD.cpp:301-396 (cdt = c * dt formed in the kernel). With RK2 / RK3
+// and L.ghost_out the stages leave their outputs ghost-filled, so only the
+// first step of a launch sequence (first) fills the ghosts of buffer 0.
+int enqueue_step(DevLoop& L, const BatchView& v, hipStream_t s, bool first) {
     int rc;
+    const bool gh = L.ghost_out;
+    const bool f0 = first || !gh;
     if ((rc = check_launch(launch_batch_dt(v, s)))) return rc;
     if (L.rkmeth == 0) {
-        if ((rc = stage(L, v, 0, 1, kRk1, 0.0, 1.0, 1.0, true, s))) return rc;
+        if ((rc = stage(L, v, 0, 1, kRk1, 0.0, 1.0, 1.0, true, true, false, s))) return rc;
         if ((rc = check_launch(launch_batch_copy(L.tiles, v, 1, 0, L.tab_x, L.n_x, s)))) return rc;
     } else if (L.rkmeth == 1) {
-        if ((rc = stage(L, v, 0, 1, kRkStage, 1.0, 0.0, 1.0, false, s))) return rc;
-        if ((rc = stage(L, v, 1, 0, kRkStage, 0.5, 0.5, 0.5, true, s))) return rc;
+        if ((rc = stage(L, v, 0, 1, kRkStage, 1.0, 0.0, 1.0, false, f0, gh, s))) return rc;
+        if ((rc = stage(L, v, 1, 0, kRkStage, 0.5, 0.5, 0.5, true, !gh, gh, s))) return rc;
     } else {
-        if ((rc = stage(L, v, 0, 1, kRkStage, 1.0, 0.0, 1.0, false, s))) return rc;
-        if ((rc = stage(L, v, 1, 2, kRkStage, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, false, s))) return rc;
-        if ((rc = stage(L, v, 2, 0, kRkStage, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, true, s))) return rc;
+        if ((rc = stage(L, v, 0, 1, kRkStage, 1.0, 0.0, 1.0, false, f0, gh, s))) return rc;
+        if ((rc = stage(L, v, 1, 2, kRkStage, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, false, !gh, gh, s))) return rc;
+        if ((rc = stage(L, v, 2, 0, kRkStage, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, true, !gh, gh, s))) return rc;
     }
     return check_launch(launch_batch_advance(v, false, s));
 }
@@ -118,7 +123,7 @@ int ensure_graph(DevLoop& L, const BatchView& v, hipStream_t s) {
     if (L.exec || L.graph_steps <= 0) return 0;
     BHIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = 0;
-    for (int k = 0; k < L.graph_steps && !rc; ++k) rc = enqueue_step(L, v, s);
+    for (int k = 0; k < L.graph_steps && !rc; ++k) rc = enqueue_step(L, v, s, k == 0);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &graph);
     if (rc) {
@@ -156,8 +161,10 @@ void devloop_free(DevLoop& L) {
 int devloop_tables(DevLoop& L, std::vector<SolveDesc>& desc, SolveDesc* desc_d, size_t part_cap) {
     std::vector<TileRef> tg, ty, tx;
     long long part_off = 0;
+    L.ghost_out = true;
     for (int s = 0; s < (int)desc.size(); ++s) {
         SolveDesc& d = desc[s];
+        if (d.g.nx < 5 || d.g.ny < 5) L.ghost_out = false;
         const long ng = ghost_blocks(d.g), ny = flux_y_blocks(d.g, L.tiles), nx = flux_x_rk_blocks(d.g, L.tiles);
         const int ntx = x_tiles_per_row(d.g, L.tiles), nty = y_tiles_per_row(d.g, L.tiles);
         for (long k = 0; k < ng; ++k) tg.push_back({s, (int)k, 0, 0});
@@ -194,8 +201,8 @@ int devloop_steps(DevLoop& L, const BatchView& v, int n, hipStream_t s) {
         if ((rc = ensure_graph(L, v, s))) return rc;
         for (; n >= L.graph_steps; n -= L.graph_steps) BHIP_TRY(hipGraphLaunch(L.exec, s));
     }
-    for (; n > 0; --n)
-        if ((rc = enqueue_step(L, v, s))) return rc;
+    for (bool first = true; n > 0; --n, first = false)
+        if ((rc = enqueue_step(L, v, s, first))) return rc;
     return 0;
 }
 

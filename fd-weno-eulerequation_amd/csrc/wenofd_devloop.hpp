@@ -23,6 +23,7 @@ struct DevLoop {
     TileRef* tab_y = nullptr;
     TileRef* tab_x = nullptr;
     int n_ghost = 0, n_y = 0, n_x = 0;
+    bool ghost_out = false;  // every solve has nx, ny >= 5: RK2 / RK3 stages write their outputs' ghosts
     int graph_steps = 16;  // steps per graph replay; 0: plain launches
     hipGraphExec_t exec = nullptr;
     double* graph_U[3] = {nullptr, nullptr, nullptr};  // state buffers the graph was captured with

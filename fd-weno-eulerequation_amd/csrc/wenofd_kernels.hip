@@ -696,13 +696,14 @@ __global__ __launch_bounds__(kYC* kYR, (y_min_waves<S, false>())) void k_batch_f
 template <int S, int XT>
 __global__ __launch_bounds__(XT, (x_min_waves<S, XT, false>())) void k_batch_flux_x_rk(BatchView b, int in, int out,
                                                                                         int mode, double a,
-                                                        double bcoef, double c_stage, int last,
+                                                        double bcoef, double c_stage, int last, int ghosts,
                                                         const TileRef* __restrict__ tiles) {
     TileRef tr;
     SolveDesc d;
     SolveState st;
     if (!batch_block(b, tiles, tr, d, st)) return;
     RkCoef k{mode, d.gravity, a, bcoef, c_stage * st.dt};
+    if (ghosts) k.gh = GhostOut{1, d.bc, b.special + d.sp_off};
     double* part = last ? b.part + d.part_off + tr.tile : nullptr;
     int* pnan = last ? b.part_nan + d.part_off + tr.tile : nullptr;
     flux_x_rk_tile<S, XT>(b.U[in] + d.u_off, b.Fy + d.fy_off, b.U[0] + d.u_off, b.U[out] + d.u_off, nullptr, d.g, k,
@@ -986,12 +987,12 @@ hipError_t launch_batch_flux_y(int scheme, const Tiles& t, const BatchView& b, i
 }
 
 hipError_t launch_batch_flux_x_rk(int scheme, const Tiles& t, const BatchView& b, int in, int out, int mode,
-                                  double a, double bcoef, double c_stage, bool last, const TileRef* tiles,
-                                  int nblocks, hipStream_t s) {
+                                  double a, double bcoef, double c_stage, bool last, bool ghosts,
+                                  const TileRef* tiles, int nblocks, hipStream_t s) {
     return with_scheme(scheme, [&](auto S) {
         with_x_tile(t.x, [&](auto XT) {
             hipLaunchKernelGGL((k_batch_flux_x_rk<S(), XT()>), dim3(nblocks), dim3(XT()), 0, s, b, in, out, mode, a,
-                               bcoef, c_stage, last ? 1 : 0, tiles);
+                               bcoef, c_stage, last ? 1 : 0, ghosts ? 1 : 0, tiles);
         });
     });
 }
