@@ -337,8 +337,10 @@ __global__ __launch_bounds__(kYC* kYR, (y_min_waves<S, HYB>())) void k_flux_y(co
 
 // Minimum over the wave's 64 lanes of values that are never NaN, returned to
 // every lane: DPP lane exchanges (quad permutations, row mirrors, then the
-// row broadcasts of lanes 15 and 31) leave the minimum in lane 63. No LDS
-// round trips; MIN is exact, so the order of the exchanges does not matter.
+// row broadcasts of lanes 15 and 31) leave the minimum in lane 63. All 64
+// lanes must be active (every caller reduces after its per-thread work has
+// reconverged). No LDS round trips; MIN is exact, so the order of the
+// exchanges does not matter.
 template <int CTRL, int ROWS = 0xF>
 __device__ __forceinline__ double dpp_f64(double v) {
     const long long b = __double_as_longlong(v);

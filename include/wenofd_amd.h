@@ -86,9 +86,10 @@ int wenofd_version(void);
  * product and sum as the reference's source writes it, so results equal the
  * reference built without FMA contraction bit for bit; 1 lets the compiler
  * contract a * b + c into FMAs, as the reference's optimised CPU build does,
- * and divide by multiplying with the reciprocal, for fewer instructions
- * (results then agree with either reference build to the few-ulp level of
- * those two builds' own difference).
+ * divides by multiplying with the reciprocal and forms the WENO weights over
+ * a common denominator (the same weights, one reciprocal per reconstruction),
+ * for fewer instructions (results then agree with either reference build to
+ * the few-ulp level of those two builds' own difference).
  */
 int wenofd_create(const char* config_text, int device, wenofd_solver** out);
 /* CConfig(path).read(option) then wenofd_create (utils/readConfig.hpp:39-80). */
