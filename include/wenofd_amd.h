@@ -14,8 +14,9 @@
  *   - A solver handle owns its device buffers and one HIP stream on one
  *     device; it is not thread-safe. Distinct handles may be used from
  *     distinct threads. Host buffers passed in are caller-owned.
- *   - All solver arithmetic is IEEE FP64 without contraction; results equal a
- *     no-contraction build of the reference bit for bit.
+ *   - All solver arithmetic is IEEE FP64; with FP_CONTRACT = 0 (the default)
+ *     it is done without contraction and results equal a no-contraction
+ *     build of the reference bit for bit (FP_CONTRACT below).
  *   - State layouts for get/set: WENOFD_LAYOUT_REF = interior points in the
  *     reference's Array2D<Cvector> order [NX][NY][4] (x outer, y inner, then
  *     rho, rho u, rho v, E); WENOFD_LAYOUT_SOA = interior [4][NY][NX];
