@@ -392,12 +392,16 @@ def main():
                                f", WENO-{args.scheme}, RK3, CFL {args.cfl}, dt from calculateDeltaT every step",
                    "nx": n, "ny": n, "scheme": args.scheme, "rk": "RK3", "cfl": args.cfl, "case": args.case,
                    "fp": args.fp + (" (FP_CONTRACT = 1: FMA contraction, as the reference's -O3 -march=native "
-                                    "build, and division by the reciprocal)" if args.fp == "fused" else
+                                    "build, division by the reciprocal, WENO weights over a common "
+                                    "denominator)" if args.fp == "fused" else
                                     " (FP_CONTRACT = 0: no contraction)")},
         "fp_modes": modes,
         "roofline": {
             "kernel": dom,
-            "bound": "fp64",
+            # the compute roof (the contract's "mfma"): these kernels issue FP64
+            # vector instructions, whose MI355X peak equals the FP64 matrix one
+            "bound": "mfma",
+            "compute": "FP64 VALU (78.6 TFLOP/s on MI355X, the FP64 MFMA peak as well)",
             "achieved": dk["tflops"],
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
