@@ -423,6 +423,10 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
     const int j = row + 4;
     const int f0 = 4 + bx * XC;  // first face = first cell of the tile
     const double* rowp = Uin + (size_t)j * g.pitch;
+    // The tile's loads and split at raised wave priority, so that a starting
+    // block issues its HBM requests ahead of the face work of the blocks
+    // around it (-4 % sweep time); the faces run at the normal priority.
+    __builtin_amdgcn_s_setprio(2);
     for (int q = t; q < XP; q += XT) {
         const int p = f0 - 3 + q;  // point p sits in slot p - f0 + 3
         if (p < g.nx + 8) {
@@ -439,6 +443,7 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
         }
     }
     __syncthreads();
+    __builtin_amdgcn_s_setprio(0);
     const int f = f0 + t;
     auto full_face = [&](int tt) {
         double Ul[4], Ur[4], Fh[4];
