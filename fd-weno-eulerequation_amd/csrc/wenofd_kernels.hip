@@ -41,6 +41,21 @@ constexpr bool kCapRecip = true;
 constexpr bool kCapRecip = false;
 #endif
 
+// Blocks are dispatched round-robin over the chip's 8 XCDs (block b runs on
+// XCD b mod 8), each with its own L2. xcd_block(b, n) maps b to a logical
+// block index so that every XCD works through one contiguous range of the n
+// logical blocks (a bijection on [0, n)): neighbouring tiles, which read the
+// same rows, then share one L2 (the x sweep: consecutive rows share a y-face
+// row; -0.7 % per step; the y sweep's tiles share only halo rows and gain
+// nothing).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int n) {
+    const int per = (n + kXcds - 1) / kXcds;           // blocks of the XCDs that take the most
+    const int tall = n % kXcds == 0 ? kXcds : n % kXcds;  // how many XCDs take `per`; the rest per - 1
+    const int xcd = b % kXcds, local = b / kXcds;
+    return xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
+}
+
 // ---------------------------------------------------------------------------
 // Ghost fill: exchangeGhostCellsValue (one-rank periodic-x ring) + setBoundary,
 // src/WENOFD.cpp:850-929 and :998-1159, restricted to the ghost points the
@@ -570,9 +585,10 @@ __global__ __launch_bounds__(XT, (x_min_waves<S, XT, HYB>())) void k_flux_x_rk(c
         if (!st->active) return;
         k.cdt = k.cdt * st->dt;
     }
-    const long b = (long)blockIdx.y * gridDim.x + blockIdx.x;
+    const int b = blockIdx.y * gridDim.x + blockIdx.x;
+    const int lb = xcd_block(b, gridDim.x * gridDim.y);  // consecutive rows of tiles on one XCD
     flux_x_rk_tile<S, XT, HYB>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part ? dt_part + b : nullptr,
-                               dt_part ? dt_nan + b : nullptr, blockIdx.x, blockIdx.y, flags);
+                               dt_part ? dt_nan + b : nullptr, lb % gridDim.x, lb / gridDim.x, flags);
 }
 
 // ---------------------------------------------------------------------------
