@@ -690,9 +690,10 @@ __global__ __launch_bounds__(kDtFinalThreads) void k_dt_final(const double* __re
 // The block's table entry, then its solve's descriptor and time state by
 // value, issued together behind the one entry load. False for a solve that is
 // no longer active.
+// idx: the table entry (default: the block id).
 __device__ __forceinline__ bool batch_block(const BatchView& b, const TileRef* __restrict__ tiles, TileRef& tr,
-                                            SolveDesc& d, SolveState& st) {
-    tr = tiles[blockIdx.x];
+                                            SolveDesc& d, SolveState& st, int idx = -1) {
+    tr = tiles[idx < 0 ? (int)blockIdx.x : idx];
     d = b.desc[tr.solve];
     st = b.st[tr.solve];
     return st.active;
@@ -724,7 +725,8 @@ __global__ __launch_bounds__(XT, (x_min_waves<S, XT, false>())) void k_batch_flu
     TileRef tr;
     SolveDesc d;
     SolveState st;
-    if (!batch_block(b, tiles, tr, d, st)) return;
+    // table entries in XCD-contiguous ranges (rows of one solve meet in one L2)
+    if (!batch_block(b, tiles, tr, d, st, xcd_block(blockIdx.x, gridDim.x))) return;
     RkCoef k{mode, d.gravity, a, bcoef, c_stage * st.dt};
     if (ghosts) k.gh = GhostOut{1, d.bc, b.special + d.sp_off};
     double* part = last ? b.part + d.part_off + tr.tile : nullptr;
