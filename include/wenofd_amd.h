@@ -22,7 +22,10 @@
  *     rho, rho u, rho v, E); WENOFD_LAYOUT_SOA = interior [4][NY][NX];
  *     WENOFD_LAYOUT_REF_GHOSTED = [(NX+8)][(NY+8)][4] including the 4 ghost
  *     points per side (corner ghosts are never read by the solver; get
- *     returns them as zero).
+ *     returns them as zero). Ghost points are filled before every RK stage
+ *     reads them; between steps they hold a fill of the state (RK2 / RK3) or
+ *     of an earlier state (RK1), where the reference's hold the fill of its
+ *     last stage's input. wenofd_set_boundary() refills them from the state.
  */
 #ifndef WENOFD_AMD_H
 #define WENOFD_AMD_H

@@ -43,6 +43,23 @@ def test_steps_leave_the_ghost_layer_filled(tc, rk, graph_steps, fp):
             assert same(G[m], F[m]), (tc, n)
 
 
+@pytest.mark.parametrize("nx,ny,rk", [(5, 5, 2), (5, 7, 1), (7, 5, 2), (9, 5, 1)])
+def test_ghost_layer_at_the_smallest_grids(nx, ny, rk):
+    """The smallest grids the library accepts (5 points a side: every ghost
+    then copies an interior point), RK2 and RK3, every boundary kind. (RK1
+    writes no ghosts: its next step fills them before they are read.)"""
+    for tc in (0, 1, 3):
+        print(f"[gpu] ghosts small tc={tc} {nx}x{ny} rk={rk}", flush=True)
+        with W.Solver(TESTCASE=tc, SCHEME=1, RKMETH=rk, CFL="0.3", ELEMNUMX=nx, ELEMNUMY=ny) as s:
+            m = _interior_ghosts(nx, ny)
+            for graph_steps in (0, 16):
+                s.set_device_loop(graph_steps)
+                s.advance(18)
+                G = s.get_state(W.LAYOUT_REF_GHOSTED)
+                s.set_boundary()
+                assert same(G[m], s.get_state(W.LAYOUT_REF_GHOSTED)[m]), (tc, graph_steps)
+
+
 def test_batch_ghost_layers_equal_the_single_solves():
     """The batch's stages write the same ghost points as the single solves'."""
     cfgs = [dict(TESTCASE=tc, SCHEME=1, RKMETH=2, CFL="0.5", ELEMNUMX=nx, ELEMNUMY=ny, **extra)
