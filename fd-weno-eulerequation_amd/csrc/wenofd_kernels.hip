@@ -236,6 +236,14 @@ __device__ __forceinline__ int compact_flagged(bool fl, int* sIdx, int* sCnt) {
     return total;
 }
 
+// The compacted slot that thread t works on: the slots start at wave `rot`
+// (wrapping round the block), so that the full fluxes of successive passes and
+// blocks land on every SIMD of a CU rather than always on waves 0 and 1.
+template <int NT>
+__device__ __forceinline__ int compact_slot(int rot) {
+    return (int)((threadIdx.x + NT - 64 * (rot % (NT / 64))) % NT);
+}
+
 // HYB: faces whose flag (flags[(f-4) nx + (i-4)]) is zero take the
 // component-wise upwind-5 flux; the flagged ones are gathered and computed
 // with the characteristic WENO flux by as few waves as they fill.
@@ -316,7 +324,8 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
                 }
             }
             const int nfl = compact_flagged<kYC * kYR>(fl, sIdx, sCnt);
-            for (int q = threadIdx.x; q < nfl; q += kYC * kYR) {
+            const int q = compact_slot<kYC * kYR>(bx + by + (f - F0) / kYR);
+            if (q < nfl) {
                 const int tt = sIdx[q];
                 full_face(tt % kYC, f + tt / kYC);
             }
@@ -488,7 +497,8 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
             }
         }
         const int nfl = compact_flagged<XT>(fl, sIdx, sCnt);
-        for (int q = t; q < nfl; q += XT) full_face(sIdx[q]);
+        const int q = compact_slot<XT>(bx + row);
+        if (q < nfl) full_face(sIdx[q]);
     } else {
         if (t <= XC && f <= g.nx + 4) full_face(t);
     }
