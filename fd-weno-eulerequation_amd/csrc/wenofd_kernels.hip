@@ -215,38 +215,12 @@ constexpr int kRing = 9;  // ring rows: a step reads 9 consecutive point rows
 // Point rows f-1 .. f+3 are read while rows f+2 .. f+5 are written: 8 slots.
 constexpr int kRawRing = 8;
 
-// Block-wide compaction of the flagged faces of one pass: slot `idx` of the
-// returned count receives the thread index of each flagged thread, in thread
-// order. Uses sCnt[blockDim/64] and sIdx[blockDim].
-template <int NT>
-__device__ __forceinline__ int compact_flagged(bool fl, int* sIdx, int* sCnt) {
-    const int t = threadIdx.x, lane = t % 64, w = t / 64;
-    const unsigned long long m = __ballot(fl);
-    const int pos = __popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) sCnt[w] = __popcll(m);
-    __syncthreads();
-    int base = 0, total = 0;
-#pragma unroll
-    for (int q = 0; q < NT / 64; ++q) {
-        base += q < w ? sCnt[q] : 0;
-        total += sCnt[q];
-    }
-    if (fl) sIdx[base + pos] = t;
-    __syncthreads();
-    return total;
-}
-
-// The compacted slot that thread t works on: the slots start at wave `rot`
-// (wrapping round the block), so that the full fluxes of successive passes and
-// blocks land on every SIMD of a CU rather than always on waves 0 and 1.
-template <int NT>
-__device__ __forceinline__ int compact_slot(int rot) {
-    return (int)((threadIdx.x + NT - 64 * (rot % (NT / 64))) % NT);
-}
-
 // HYB: faces whose flag (flags[(f-4) nx + (i-4)]) is zero take the
-// component-wise upwind-5 flux; the flagged ones are gathered and computed
-// with the characteristic WENO flux by as few waves as they fill.
+// component-wise upwind-5 flux; the flagged ones take the characteristic WENO
+// flux in their own lanes, so a wave runs it only if one of its 64 faces is
+// flagged. (Gathering the flagged faces of a step onto the fewest waves makes
+// the WENO flux's column a per-lane value: 182 VGPRs, 2 waves per SIMD, and a
+// slower sweep at every flag rate measured.)
 template <int S, int YF, bool HYB = false>
 __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double* __restrict__ Fy, const Geom& g,
                                             int bx, int by, const unsigned char* __restrict__ flags = nullptr) {
@@ -254,8 +228,6 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
     __shared__ double sP[4][kRing][kYC];
     __shared__ double sM[4][kRing][kYC];
     __shared__ double sU[4][kRawRing][kYC];
-    __shared__ int sIdx[HYB ? kYC * kYR : 1];
-    __shared__ int sCnt[kYR];
     const int tx = threadIdx.x % kYC;
     // one wave = one face row of a step: ty (and every ring row index derived
     // from it) is wave-uniform, which readfirstlane makes scalar arithmetic
@@ -323,12 +295,7 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
                     for (int r = 0; r < 4; ++r) Fy[r * g.fy_plane + o] = Fh[r];
                 }
             }
-            const int nfl = compact_flagged<kYC * kYR>(fl, sIdx, sCnt);
-            const int q = compact_slot<kYC * kYR>(bx + by + (f - F0) / kYR);
-            if (q < nfl) {
-                const int tt = sIdx[q];
-                full_face(tt % kYC, f + tt / kYC);
-            }
+            if (fl) full_face(tx, fr);
         } else {
             if (col_ok && fr < F1) full_face(tx, fr);
         }
@@ -483,8 +450,6 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
         for (int r = 0; r < 4; ++r) sF[r][tt] = Fh[r];
     };
     if constexpr (HYB) {
-        __shared__ int sIdx[XT];
-        __shared__ int sCnt[W];
         bool fl = false;
         if (t <= XC && f <= g.nx + 4) {
             fl = flags[(size_t)row * (g.nx + 1) + (f - 4)] != 0;
@@ -496,9 +461,7 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
                 for (int r = 0; r < 4; ++r) sF[r][t] = Fh[r];
             }
         }
-        const int nfl = compact_flagged<XT>(fl, sIdx, sCnt);
-        const int q = compact_slot<XT>(bx + row);
-        if (q < nfl) full_face(sIdx[q]);
+        if (fl) full_face(t);
     } else {
         if (t <= XC && f <= g.nx + 4) full_face(t);
     }
