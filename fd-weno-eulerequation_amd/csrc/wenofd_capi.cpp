@@ -201,7 +201,9 @@ int stage(wenofd_solver* h, double* in, const double* un, double* out, const RkC
 // dilated by sw_every faces in both directions of the face grid: the flags are
 // computed from the initial state of every sw_every-th step and serve all
 // stages of that many steps (a wave moves less than one cell per step at
-// CFL <= 1).
+// CFL <= 1). The y faces' flags are then shared by each aligned group of 64
+// columns of a face row, the faces one wave of the y sweep computes: a wave
+// with one flagged face pays the WENO flux anyway, and so skips the cheap one.
 int classify(wenofd_solver* h, float* logits_x = nullptr, float* logits_y = nullptr, bool dilate = true,
              bool force = true) {
     int rc;
@@ -215,8 +217,10 @@ int classify(wenofd_solver* h, float* logits_x = nullptr, float* logits_y = null
         hipError_t e = launch_classify(0, h->U, h->g, h->sw_params, h->sw_thr, h->fl_raw_x, logits_x, h->stream);
         if (e == hipSuccess)
             e = launch_classify(1, h->U, h->g, h->sw_params, h->sw_thr, h->fl_raw_y, logits_y, h->stream);
-        if (e == hipSuccess && dilate) e = launch_dilate(h->fl_raw_x, h->fl_x, h->g.ny, h->g.nx + 1, rad, h->stream);
-        if (e == hipSuccess && dilate) e = launch_dilate(h->fl_raw_y, h->fl_y, h->g.ny + 1, h->g.nx, rad, h->stream);
+        if (e == hipSuccess && dilate)
+            e = launch_dilate(h->fl_raw_x, h->fl_x, h->g.ny, h->g.nx + 1, rad, false, h->stream);
+        if (e == hipSuccess && dilate)
+            e = launch_dilate(h->fl_raw_y, h->fl_y, h->g.ny + 1, h->g.nx, rad, true, h->stream);
         return e;
     });
 }

@@ -148,7 +148,8 @@ hipError_t launch_classify(int dir, const double* U, const Geom& g, const float*
 // Features [faces][4][10] and labels [faces][4] of every face of direction dir.
 hipError_t launch_harvest(int dir, const double* U, const Geom& g, float* X, unsigned char* Y, hipStream_t s);
 // OR over the (2 rad + 1)^2 neighbourhood of each face in the rows x cols face grid.
-hipError_t launch_dilate(const unsigned char* in, unsigned char* out, int rows, int cols, int rad, hipStream_t s);
+hipError_t launch_dilate(const unsigned char* in, unsigned char* out, int rows, int cols, int rad, bool group,
+                         hipStream_t s);
 hipError_t launch_fill_u8(unsigned char* p, long n, unsigned char v, hipStream_t s);
 hipError_t launch_mlp_infer(const float* P, const float* X, long n, float* z, hipStream_t s);
 hipError_t launch_keep_count(const unsigned char* Y, long n, unsigned long long seed, double p_pos, double p_neg,

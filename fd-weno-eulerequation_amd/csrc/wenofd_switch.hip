@@ -255,22 +255,29 @@ __global__ __launch_bounds__(kSwT) void k_switch_faces(const double* __restrict_
     }
 }
 
-// out[r][c] = OR of in over the (2 rad + 1)^2 neighbourhood (rows x cols grid of faces).
+// out[r][c] = OR of in over the (2 rad + 1)^2 neighbourhood (rows x cols grid
+// of faces). One wave per aligned group of 64 columns of a row; with `group`,
+// every face of a group takes the OR of the group's dilated flags.
 __global__ __launch_bounds__(256) void k_dilate(const unsigned char* __restrict__ in, unsigned char* __restrict__ out,
-                                                int rows, int cols, int rad) {
-    const long k = (long)blockIdx.x * 256 + threadIdx.x;
-    if (k >= (long)rows * cols) return;
-    const int r = (int)(k / cols), c = (int)(k % cols);
-    unsigned char v = 0;
-    for (int dr = -rad; dr <= rad; ++dr) {
-        const int rr = r + dr;
-        if (rr < 0 || rr >= rows) continue;
-        for (int dc = -rad; dc <= rad; ++dc) {
-            const int cc = c + dc;
-            if (cc >= 0 && cc < cols) v |= in[(size_t)rr * cols + cc];
+                                                int rows, int cols, int rad, int group) {
+    const int groups = (cols + 63) / 64;
+    const long wv = (long)blockIdx.x * 4 + threadIdx.x / 64;  // wave-uniform
+    if (wv >= (long)rows * groups) return;
+    const int r = (int)(wv / groups), c = (int)(wv % groups) * 64 + (int)(threadIdx.x % 64);
+    const bool ok = c < cols;
+    bool v = false;
+    if (ok) {
+        for (int dr = -rad; dr <= rad; ++dr) {
+            const int rr = r + dr;
+            if (rr < 0 || rr >= rows) continue;
+            for (int dc = -rad; dc <= rad; ++dc) {
+                const int cc = c + dc;
+                if (cc >= 0 && cc < cols) v = v || in[(size_t)rr * cols + cc] != 0;
+            }
         }
     }
-    out[k] = v ? 1 : 0;
+    if (group) v = __any(v);
+    if (ok) out[(size_t)r * cols + c] = v ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_fill_u8(unsigned char* __restrict__ p, long n, unsigned char v) {
@@ -502,9 +509,10 @@ hipError_t launch_harvest(int dir, const double* U, const Geom& g, float* X, uns
     return hipGetLastError();
 }
 
-hipError_t launch_dilate(const unsigned char* in, unsigned char* out, int rows, int cols, int rad, hipStream_t s) {
-    const long n = (long)rows * cols;
-    hipLaunchKernelGGL(k_dilate, dim3((n + 255) / 256), dim3(256), 0, s, in, out, rows, cols, rad);
+hipError_t launch_dilate(const unsigned char* in, unsigned char* out, int rows, int cols, int rad, bool group,
+                         hipStream_t s) {
+    const long waves = (long)rows * ((cols + 63) / 64);
+    hipLaunchKernelGGL(k_dilate, dim3((waves + 3) / 4), dim3(256), 0, s, in, out, rows, cols, rad, group ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -179,7 +179,9 @@ int wenofd_set_tiles(wenofd_solver* h, int x_faces, int y_faces);
  * inference, FP32 accumulate) reads the normalised 5-point stencils of each
  * characteristic field of each face and flags the faces whose WENO-Z weights
  * it predicts to leave the linear weights (logit > threshold for any field).
- * The flags are dilated by one face and serve all stages of the step: flagged
+ * The flags are dilated by one face (the y faces' flags are then shared by
+ * each aligned group of 64 columns of a face row, the faces one wave of the y
+ * sweep computes) and serve all stages of the step: flagged
  * faces take the reference's characteristic WENO flux, the others the cheap
  * component-wise linear upwind-5 flux. The plain solver is the fallback: a
  * switched step that leads to a NaN time step is redone without the switch.

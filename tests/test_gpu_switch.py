@@ -108,6 +108,24 @@ def test_mlp_inference_matches_oracle():
     g.close()
 
 
+def group64(f):
+    """The y faces' flags after dilation: each aligned group of 64 columns of a
+    face row takes the OR of its flags."""
+    out = f.copy()
+    for c in range(0, f.shape[1], 64):
+        out[:, c:c + 64] = f[:, c:c + 64].any(axis=1, keepdims=True)
+    return out
+
+
+def test_group64_of_the_y_flags():
+    f = np.zeros((3, 130), np.uint8)
+    f[0, 63] = f[1, 64] = f[2, 129] = 1
+    g = group64(f)
+    assert g[0, :64].all() and not g[0, 64:].any()
+    assert not g[1, :64].any() and g[1, 64:128].all() and not g[1, 128:].any()
+    assert not g[2, :128].any() and g[2, 128:].all()
+
+
 def test_classifier_logits_flags_and_dilation():
     g, _ = pair(5, 45, 38, seed=5)
     g.step()
@@ -130,7 +148,7 @@ def test_classifier_logits_flags_and_dilation():
         return np.max([p[1 + a:1 + a + f.shape[0], 1 + b:1 + b + f.shape[1]] for a in (-1, 0, 1) for b in (-1, 0, 1)],
                       axis=0)
 
-    assert same(dx, dil(fx)) and same(dy, dil(fy))
+    assert same(dx, dil(fx)) and same(dy, group64(dil(fy)))
     assert 0 < fx.mean() < 1
     g.close()
 
@@ -141,19 +159,21 @@ def test_switched_steps_match_oracle_with_the_same_flags(rk):
     P = random_params(4, scale=2.0)
     g.switch_set(W.SWITCH_MLP, P, 0.0)
     _, _, lx, ly = g.switch_classify(dilate=False, logits=True)
-    # a threshold that leaves about 10% of the faces flagged before dilation
-    thr = float(np.quantile(np.concatenate([lx.max(axis=2).ravel(), ly.max(axis=2).ravel()]), 0.9))
+    # a threshold that leaves about 0.5 % of the faces flagged before dilation,
+    # so that some face rows' y flags (shared by the 34 columns) stay clear
+    thr = float(np.quantile(np.concatenate([lx.max(axis=2).ravel(), ly.max(axis=2).ravel()]), 0.995))
     g.switch_set(W.SWITCH_MLP, P, thr)
-    mixed = []
+    mixed_x, mixed_y = [], []
     for _ in range(3):
         dt = g.step()
         fx, fy = g.switch_flags()
-        mixed.append(0 < fx.mean() < 1 and 0 < fy.mean() < 1)
+        mixed_x.append(0 < fx.mean() < 1)
+        mixed_y.append(0 < fy.mean() < 1)
         o.set_flags(fx, fy)
         assert o.delta_t() == dt
         o.step(dt)
         assert same(g.get_state(), o.get_state())
-    assert all(mixed)
+    assert any(mixed_x) and any(mixed_y)  # both fluxes ran in both directions
     g.close()
 
 
@@ -319,7 +339,7 @@ def test_dilation_radius_below_the_interval():
         dt = g.step()
         fx, fy = g.switch_flags()
         if k == 0:
-            assert same(fx, dil(rx, 1)) and same(fy, dil(ry, 1))
+            assert same(fx, dil(rx, 1)) and same(fy, group64(dil(ry, 1)))
         seen.append(fx)
         o.set_flags(fx, fy)
         assert o.delta_t() == dt
@@ -347,7 +367,7 @@ def test_classification_interval_reuses_dilated_flags():
         dt = g.step()
         fx, fy = g.switch_flags()
         if k == 0:
-            assert same(fx, dil(rx, 2)) and same(fy, dil(ry, 2))
+            assert same(fx, dil(rx, 2)) and same(fy, group64(dil(ry, 2)))
         seen.append((fx, fy))
         o.set_flags(fx, fy)
         assert o.delta_t() == dt
