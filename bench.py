@@ -217,14 +217,14 @@ def batched(W, steps, seq_steps, fp, seed=0):
     return out
 
 
-HYBRID_CONFIGS = ((1, 1), (8, 2))  # (classification interval, dilation radius)
+HYBRID_CONFIGS = ((1, 1), (8, 2), (16, 3))  # (classification interval, dilation radius)
 
 
 def hybrid(W, n, seed, t_end, truth, fp):
     """Learned switch against plain WENO-Z on an unseen seeded four-quadrant
     problem (seed outside the training seeds 0..255), RK3 CFL 0.6 to t_end,
-    each run timed whole: classification every step with 3 x 3 dilation, and
-    every 8 steps with 5 x 5 dilation. With truth, all are also compared with
+    each run timed whole: classification every step with 3 x 3 dilation, every
+    8 steps with 5 x 5 and every 16 steps with 7 x 7. With truth, all are also compared with
     plain WENO-Z on the nested (2n-1)^2 grid sampled at even indices."""
     P, thr = W.load_switch()
     cfg = W.config_text(TESTCASE=5, SCHEME=1, RKMETH=2, CFL=0.6, ELEMNUMX=n, ELEMNUMY=n, QUAD_SEED=seed,
