@@ -177,6 +177,31 @@ def test_switched_steps_match_oracle_with_the_same_flags(rk):
     g.close()
 
 
+@pytest.mark.parametrize("scheme", [0, 2, 3])
+def test_switched_steps_of_every_scheme_match_oracle(scheme):
+    """The switched sweeps of WENO-JS, -ZP and -ZPI (flagged faces take that
+    scheme's characteristic flux, the others upwind-5) equal the oracle's
+    hybrid steps bit for bit, given the same flags."""
+    seed = 9
+    print(f"[gpu] switched steps scheme={scheme} 34x29 seed={seed}", flush=True)
+    g = W.Solver(cfg(5, scheme, 2, 0.5, 34, 29, QUAD_SEED=seed))
+    o = O.Solver(5, scheme, 2, 0.5, 34, 29, t_end=100.0, quad=O.seeded_quadrant(seed))
+    P, thr = W.load_switch()
+    g.switch_set(W.SWITCH_MLP, P, thr)
+    mixed_x, mixed_y = [], []
+    for _ in range(3):
+        dt = g.step()
+        fx, fy = g.switch_flags()
+        mixed_x.append(0 < fx.mean() < 1)
+        mixed_y.append(0 < fy.mean() < 1)
+        o.set_flags(fx, fy)
+        assert o.delta_t() == dt
+        o.step(dt)
+        assert same(g.get_state(), o.get_state()), scheme
+    assert any(mixed_x) and any(mixed_y)
+    g.close()
+
+
 def nan_time_step(U, gamma=1.4):
     """Whether any point's local CFL step is NaN (the library's NaN test; the
     reference's own minimum loop can lose a NaN that a finite value follows)."""
