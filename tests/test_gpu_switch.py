@@ -177,6 +177,31 @@ def test_switched_steps_match_oracle_with_the_same_flags(rk):
     g.close()
 
 
+@pytest.mark.parametrize("tc,nx,ny", [(0, 70, 29), (3, 30, 72), (4, 70, 33)])
+def test_switched_steps_of_every_boundary_kind_match_oracle(tc, nx, ny):
+    """Switched RK3 steps on periodic (SMOOTH, VORTEX) and slip + SPECIAL
+    (RTI) boundaries equal the oracle's hybrid steps bit for bit, given the
+    same flags (random network, threshold at the 99.5 % logit quantile)."""
+    g, o = pair(tc, nx, ny, rk=2)
+    P = random_params(6, scale=2.0)
+    g.switch_set(W.SWITCH_MLP, P, 0.0)
+    _, _, lx, ly = g.switch_classify(dilate=False, logits=True)
+    thr = float(np.quantile(np.concatenate([lx.max(axis=2).ravel(), ly.max(axis=2).ravel()]), 0.995))
+    g.switch_set(W.SWITCH_MLP, P, thr)
+    mixed_x, mixed_y = [], []
+    for _ in range(3):
+        dt = g.step()
+        fx, fy = g.switch_flags()
+        mixed_x.append(0 < fx.mean() < 1)
+        mixed_y.append(0 < fy.mean() < 1)
+        o.set_flags(fx, fy)
+        assert o.delta_t() == dt
+        o.step(dt)
+        assert same(g.get_state(), o.get_state()), tc
+    assert any(mixed_x) and any(mixed_y)
+    g.close()
+
+
 @pytest.mark.parametrize("scheme", [0, 2, 3])
 def test_switched_steps_of_every_scheme_match_oracle(scheme):
     """The switched sweeps of WENO-JS, -ZP and -ZPI (flagged faces take that
