@@ -167,18 +167,18 @@ def test_fused_device_loop_equals_host_loop(tc, rk):
         assert same(a.get_state(), b.get_state()) and a.info().now == b.info().now
 
 
-@pytest.mark.parametrize("mode", ["weno", "linear", "mlp"])
-def test_fused_switched_steps_match_oracle_with_the_same_flags(mode):
+@pytest.mark.parametrize("mode,scheme", [("weno", 1), ("linear", 1), ("mlp", 1), ("mlp", 0), ("mlp", 2), ("mlp", 3)])
+def test_fused_switched_steps_match_oracle_with_the_same_flags(mode, scheme):
     """The learned switch's sweeps in the contracted mode: each switched step,
     started from the GPU's own state, agrees with the oracle's hybrid step with
-    the same flags (all WENO, all upwind-5, or the trained network's) to the
-    rounding floor."""
+    the same flags (all WENO, all upwind-5, or the trained network's; flagged
+    faces of every WENO scheme) to the rounding floor."""
     import oracle as O
 
-    base = dict(TESTCASE=5, SCHEME=1, RKMETH=2, CFL="0.5", ELEMNUMX=34, ELEMNUMY=29, QUAD_SEED=9, TEND=100,
+    base = dict(TESTCASE=5, SCHEME=scheme, RKMETH=2, CFL="0.5", ELEMNUMX=34, ELEMNUMY=29, QUAD_SEED=9, TEND=100,
                 FP_CONTRACT=1)
-    print(f"[gpu] fused switched steps, flags {mode}", flush=True)
-    o = O.Solver(5, 1, 2, 0.5, 34, 29, t_end=100.0, quad=O.seeded_quadrant(9))
+    print(f"[gpu] fused switched steps, flags {mode}, scheme {scheme}", flush=True)
+    o = O.Solver(5, scheme, 2, 0.5, 34, 29, t_end=100.0, quad=O.seeded_quadrant(9))
     with W.Solver(**base) as g:
         if mode == "weno":
             g.switch_set(W.SWITCH_WENO)
