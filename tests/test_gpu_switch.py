@@ -153,7 +153,7 @@ def test_classifier_logits_flags_and_dilation():
     g.close()
 
 
-@pytest.mark.parametrize("rk", [1, 2])
+@pytest.mark.parametrize("rk", [0, 1, 2])
 def test_switched_steps_match_oracle_with_the_same_flags(rk):
     g, o = pair(5, 34, 29, rk=rk, seed=9)
     P = random_params(4, scale=2.0)
