@@ -90,6 +90,18 @@ def test_all_flagged_equals_plain_and_all_linear_matches_oracle(rk):
         s.close()
 
 
+@pytest.mark.parametrize("fp", [0, 1])
+def test_all_flagged_equals_plain_full_size(fp):
+    """At 2048^2 (every tile row and column of the switched sweeps in use), the
+    switched kernels with every face flagged give the plain solver's bits."""
+    keys = dict(TESTCASE=5, SCHEME=1, RKMETH=2, CFL="0.6", ELEMNUMX=2048, ELEMNUMY=2048, QUAD_SEED=1000,
+                TEND=100, FP_CONTRACT=fp)
+    print(f"[gpu] all flagged vs plain 2048x2048 fp={fp}", flush=True)
+    with W.Solver(**keys) as a, W.Solver(**keys) as b:
+        b.switch_set(W.SWITCH_WENO)
+        assert a.advance(4) == 4 and b.advance(4) == 4
+        assert same(a.get_state(), b.get_state())
+
 def test_mlp_inference_matches_oracle():
     g, _ = pair(5, 40, 36, seed=11)
     X = np.concatenate([g.harvest(0)[0].reshape(-1, 10), g.harvest(1)[0].reshape(-1, 10)])
