@@ -102,6 +102,27 @@ def test_all_flagged_equals_plain_full_size(fp):
         assert a.advance(4) == 4 and b.advance(4) == 4
         assert same(a.get_state(), b.get_state())
 
+def test_switched_step_full_size_matches_oracle():
+    """One switched RK3 step at 2048^2 with the trained network's flags (after
+    30 steps, so that the flags follow the developing waves) equals the
+    oracle's hybrid step from the same state with the same flags."""
+    n, seed = 2048, 1000
+    print(f"[gpu] switched step vs oracle {n}x{n} seed={seed}", flush=True)
+    g = W.Solver(cfg(5, 1, 2, 0.6, n, n, QUAD_SEED=seed))
+    P, thr = W.load_switch()
+    g.switch_set(W.SWITCH_MLP, P, thr)
+    assert g.advance(30) == 30
+    o = O.Solver(5, 1, 2, 0.6, n, n, t_end=100.0, quad=O.seeded_quadrant(seed))
+    o.set_state(g.get_state())
+    dt = g.step()
+    fx, fy = g.switch_flags()
+    assert 0 < fx.mean() < 1 and 0 < fy.mean() < 1
+    o.set_flags(fx, fy)
+    assert o.delta_t() == dt
+    o.step(dt)
+    assert same(g.get_state(), o.get_state())
+    g.close()
+
 def test_mlp_inference_matches_oracle():
     g, _ = pair(5, 40, 36, seed=11)
     X = np.concatenate([g.harvest(0)[0].reshape(-1, 10), g.harvest(1)[0].reshape(-1, 10)])
