@@ -232,6 +232,26 @@ def test_periodic_twin_nodes_and_conservation_2048():
     assert np.all(np.abs(m1 - m0) <= 1e-12 * np.abs(U[:-1, :-1]).sum(axis=(0, 1)))
 
 
+@pytest.mark.parametrize("fp", [0, 1])
+def test_headline_step_2048_matches_oracle(fp):
+    """The headline workload itself (2048^2 isentropic vortex, WENO-Z, RK3, CFL
+    0.6): after 20 device-loop steps, one more step from that state equals the
+    oracle's step bit for bit without contraction, and to the rounding floor
+    (1e-13 relative max-norm) with it."""
+    n = 2048
+    with solver(4, 1, 2, 0.6, n, n, TEND=100, FP_CONTRACT=fp) as s:
+        assert s.advance(20) == 20
+        o = O.Solver(4, 1, 2, 0.6, n, n, t_end=100.0)
+        o.set_state(s.get_state())
+        dt = s.step()
+        assert o.delta_t() == dt
+        o.step(dt)
+        U = s.get_state()
+    if fp == 0:
+        assert same(U, o.get_state())
+    else:
+        assert relmax(U, o.get_state()) <= 1e-13
+
 def test_diagonal_symmetry_2048():
     """The four-quadrant states are symmetric under x <-> y with u <-> v; the x
     and y sweeps use mirrored formulas, so the solution stays symmetric to
