@@ -232,16 +232,18 @@ def test_periodic_twin_nodes_and_conservation_2048():
     assert np.all(np.abs(m1 - m0) <= 1e-12 * np.abs(U[:-1, :-1]).sum(axis=(0, 1)))
 
 
-@pytest.mark.parametrize("fp", [0, 1])
-def test_headline_step_2048_matches_oracle(fp):
-    """The headline workload itself (2048^2 isentropic vortex, WENO-Z, RK3, CFL
-    0.6): after 20 device-loop steps, one more step from that state equals the
-    oracle's step bit for bit without contraction, and to the rounding floor
-    (1e-13 relative max-norm) with it."""
-    n = 2048
-    with solver(4, 1, 2, 0.6, n, n, TEND=100, FP_CONTRACT=fp) as s:
+@pytest.mark.parametrize("tc,n,scheme,fp", [(4, 2048, 1, 0), (4, 2048, 1, 1), (4, 4096, 1, 0), (5, 2048, 2, 0)])
+def test_full_size_step_matches_oracle(tc, n, scheme, fp):
+    """Full-size workloads (the 2048^2 isentropic vortex with WENO-Z that the
+    bench measures, the 4096^2 vortex, a seeded 2048^2 four-quadrant problem
+    with WENO-ZP; RK3, CFL 0.6): after 20 device-loop steps, one more step from
+    that state equals the oracle's step bit for bit without contraction, and to
+    the rounding floor (1e-13 relative max-norm) with it."""
+    extra = {"QUAD_SEED": 1000} if tc == 5 else {}
+    quad = O.seeded_quadrant(1000) if tc == 5 else None
+    with solver(tc, scheme, 2, 0.6, n, n, TEND=100, FP_CONTRACT=fp, **extra) as s:
         assert s.advance(20) == 20
-        o = O.Solver(4, 1, 2, 0.6, n, n, t_end=100.0)
+        o = O.Solver(tc, scheme, 2, 0.6, n, n, t_end=100.0, quad=quad)
         o.set_state(s.get_state())
         dt = s.step()
         assert o.delta_t() == dt
