@@ -34,7 +34,8 @@ sys.path.insert(0, PKG)
 METRIC = "WENO-Z RK3 cell-updates/s (FP64, 2048²) + learned-hybrid speedup at equal L1"
 SCHEMES = {"JS": 0, "Z": 1, "ZP": 2, "ZPI": 3}
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
-FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector and matrix)
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector peak
+STEP_BYTES_PER_CELL_UPDATE = 256  # RK3: Un, U1, U2 read and U1, U2, Un+1 written once, 4 doubles each
 
 
 def log(msg):
@@ -116,6 +117,7 @@ def valu_issue(kernel, n, avg_us, fp):
 
 
 REF_EXE = os.path.join(ROOT, "oracle", "_ref", "ref_native")
+CPU_SHARE = 16  # host cores per GPU on the MI355X boxes
 MPIEXEC = "/opt/conda/bin/mpiexec"
 
 
@@ -138,13 +140,21 @@ def _ref_bench(cmd, workdir):
     raise RuntimeError("no timing line from the reference build")
 
 
+def _cpu_facts():
+    """CPU model, nproc and the CPU share of one GPU on this host."""
+    nproc = os.cpu_count() or 1
+    return {"host": _host_cpu(), "nproc": nproc, "cpu_share": min(nproc, CPU_SHARE)}
+
+
 def cpu_baseline(nx, ny, scheme, cfl, steps, mpi_ranks):
     """The reference's own CPU path (its sources compiled with the makefile's
     -O3 -march=native into oracle/_ref/ref_native, driven through its class
-    surface) timed on this host over a bounded sample of the same workload,
-    serially and under mpiexec; the serial CPU oracle when that build is
+    surface) timed on this host over a bounded sample of the same workload:
+    serially, and under mpiexec at each rank count of mpi_ranks (capped at the
+    host's CPU share of one GPU); the serial CPU oracle when that build is
     absent."""
     sname = [k for k, v in SCHEMES.items() if v == scheme][0]
+    facts = _cpu_facts()
     if os.path.exists(REF_EXE):
         args = ["bench", "4", str(scheme), "2", "%.17g" % cfl, str(nx), str(ny), str(steps)]
         try:
@@ -153,11 +163,17 @@ def cpu_baseline(nx, ny, scheme, cfl, steps, mpi_ranks):
                 rec = {"value": nx * ny * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
                        "sample": f"{steps} steps (calculateDeltaT + RunRK3) of the same {nx}x{ny} vortex, WENO-"
                                  f"{sname}, the reference's CWENOFD built with its makefile flags, one MPI rank, "
-                                 f"{el:.1f} s", "host": _host_cpu()}
-                if mpi_ranks > 1 and os.path.exists(MPIEXEC):
-                    el_m = _ref_bench([MPIEXEC, "-n", str(mpi_ranks), REF_EXE] + args, wd)
-                    rec["mpi"] = {"ranks": mpi_ranks, "cores": mpi_ranks, "value": nx * ny * steps / el_m,
-                                  "seconds": el_m}
+                                 f"{el:.1f} s"}
+                rec.update(facts)
+                runs = []
+                for ranks in sorted({min(r, facts["cpu_share"]) for r in mpi_ranks if r > 1}):
+                    if not os.path.exists(MPIEXEC):
+                        break
+                    log(f"cpu baseline: {steps} reference steps on {ranks} MPI ranks")
+                    el_m = _ref_bench([MPIEXEC, "-n", str(ranks), REF_EXE] + args, wd)
+                    runs.append({"ranks": ranks, "cores": ranks, "value": nx * ny * steps / el_m, "seconds": el_m})
+                if runs:
+                    rec["mpi"] = runs
                 return rec
         except (subprocess.SubprocessError, RuntimeError, OSError, ValueError, IndexError) as e:
             log(f"cpu baseline: reference build failed ({e}); timing the oracle instead")
@@ -169,9 +185,26 @@ def cpu_baseline(nx, ny, scheme, cfl, steps, mpi_ranks):
     for _ in range(steps):
         s.step(s.delta_t())
     el = time.perf_counter() - t0
-    return {"value": nx * ny * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} steps (calculateDeltaT + RunRK3) of the same {nx}x{ny} vortex, WENO-"
-                      f"{sname}, serial C++ oracle, {el:.1f} s", "host": _host_cpu()}
+    rec = {"value": nx * ny * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+           "sample": f"{steps} steps (calculateDeltaT + RunRK3) of the same {nx}x{ny} vortex, WENO-"
+                     f"{sname}, serial C++ oracle, {el:.1f} s"}
+    rec.update(facts)
+    return rec
+
+
+def copy_calibration(nbytes):
+    """A plain device copy of nbytes (torch's copy kernel: nbytes read, nbytes
+    written), for scaling the HBM counters of a profiled run: FETCH_SIZE and
+    WRITE_SIZE of this kernel against its known bytes."""
+    import torch
+
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 8, "launches": 5}), flush=True)
 
 
 def batched(W, steps, seq_steps, fp, seed=0):
@@ -304,7 +337,12 @@ def main():
     ap.add_argument("--other-steps", type=int, default=100,
                     help="steps timed in the other FP mode for comparison (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=2, help="reference CPU steps for cpu_baseline (0: skip)")
-    ap.add_argument("--cpu-ranks", type=int, default=8, help="MPI ranks of the reference's parallel CPU run")
+    ap.add_argument("--cpu-ranks", default="8,16",
+                    help="MPI rank counts of the reference's parallel CPU runs (capped at the CPU share of one GPU)")
+    ap.add_argument("--graph-steps", type=int, default=16,
+                    help="steps per graph replay of the device loops (0: every step launched from the host)")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (HBM counter calibration) and exit")
     ap.add_argument("--batch-steps", type=int, default=200, help="steps of the 64-solve batch (0: skip)")
     ap.add_argument("--seq-steps", type=int, default=20, help="steps per solve of the sequential comparison")
     ap.add_argument("--hybrid-size", type=int, default=2048, help="grid of the learned-switch comparison (0: skip)")
@@ -317,9 +355,15 @@ def main():
         print("bench.py: this solver runs one whole domain on one GPU; --gpus must be 1", file=sys.stderr)
         return 2
 
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return 0
+
     import wenofd as W
 
     n, sc = args.size, SCHEMES[args.scheme]
+    if args.graph_steps != 16:
+        W.DEFAULT_GRAPH_STEPS = args.graph_steps
 
     def make_solver(fp):
         keys = dict(SCHEME=sc, RKMETH=2, CFL=args.cfl, ELEMNUMX=n, ELEMNUMY=n, TEND=1e9, FP_CONTRACT=FP_MODES[fp])
@@ -398,16 +442,25 @@ def main():
         "fp_modes": modes,
         "roofline": {
             "kernel": dom,
-            # the compute roof (the contract's "mfma"): these kernels issue FP64
-            # vector instructions, whose MI355X peak equals the FP64 matrix one
-            "bound": "mfma",
-            "compute": "FP64 VALU (78.6 TFLOP/s on MI355X, the FP64 MFMA peak as well)",
+            # the binding roof is the FP64 vector ALU: no kernel of the plain
+            # path issues a matrix instruction, and at ~24 FLOP per byte of
+            # compulsory traffic the step sits above the HBM ridge point
+            "bound": "fp64-valu",
+            "compute": "FP64 VALU, 78.6 TFLOP/s on MI355X (FMA counted as 2 FLOP)",
             "achieved": dk["tflops"],
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": dk["tflops"] / FP64_PEAK_TFLOPS,
             "traffic": measured_traffic(dom, n, args.fp) if (args.scheme == "Z" and args.case == "vortex") else None,
-            "hbm": {"achieved": dk["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dk["gbs"] / HBM_PEAK_GBS},
+            # step level: the compulsory 256 B per cell-update (RK3: read Un,
+            # U1, U2 and write U1, U2, Un+1 once each) over the step time
+            "hbm": {"achieved": STEP_BYTES_PER_CELL_UPDATE * n * n / (el / args.steps) / 1e9, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "bytes_per_cell_update": STEP_BYTES_PER_CELL_UPDATE,
+                    "frac": STEP_BYTES_PER_CELL_UPDATE * n * n / (el / args.steps) / 1e9 / HBM_PEAK_GBS},
+            # the dominant kernel's own design bytes (its reads and writes as
+            # written, per launch) over its duration
+            "kernel_design_bytes": {"bytes_per_launch": dk["bytes_per_launch"], "achieved": dk["gbs"],
+                                    "unit": "GB/s", "frac": dk["gbs"] / HBM_PEAK_GBS},
             "valu_issue": (valu_issue(dom, n, dk["avg_us"], args.fp)
                            if (args.scheme == "Z" and args.case == "vortex") else None),
             "step_fp64_tflops": step_flops / (el / args.steps) / 1e12,
@@ -419,8 +472,9 @@ def main():
     if args.hybrid_size > 0:
         rec["hybrid"] = hybrid(W, args.hybrid_size, args.hybrid_seed, args.hybrid_tend, args.hybrid_truth, args.fp)
     if args.cpu_steps > 0:
-        log(f"cpu baseline: {args.cpu_steps} reference steps at {n}x{n}, serial and {args.cpu_ranks} MPI ranks")
-        rec["cpu_baseline"] = cpu_baseline(n, n, sc, args.cfl, args.cpu_steps, args.cpu_ranks)
+        ranks = [int(r) for r in args.cpu_ranks.split(",") if r]
+        log(f"cpu baseline: {args.cpu_steps} reference steps at {n}x{n}, serial and on {ranks} MPI ranks")
+        rec["cpu_baseline"] = cpu_baseline(n, n, sc, args.cfl, args.cpu_steps, ranks)
     else:
         rec["cpu_baseline"] = None
     print(json.dumps(rec), flush=True)

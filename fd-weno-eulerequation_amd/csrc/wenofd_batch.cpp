@@ -118,8 +118,8 @@ int enqueue_step(DevLoop& L, const BatchView& v, hipStream_t s, bool first) {
 }
 
 int ensure_graph(DevLoop& L, const BatchView& v, hipStream_t s) {
-    if (L.exec && (L.graph_U[0] != v.U[0] || L.graph_U[1] != v.U[1] || L.graph_U[2] != v.U[2]))
-        devloop_drop_graph(L);  // the graph holds the buffer addresses it was captured with
+    const GraphKey key = devloop_key(L, v);
+    if (L.exec && !(L.key == key)) devloop_drop_graph(L);  // the graph holds what it was captured with
     if (L.exec || L.graph_steps <= 0) return 0;
     BHIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = 0;
@@ -137,13 +137,24 @@ int ensure_graph(DevLoop& L, const BatchView& v, hipStream_t s) {
         L.exec = nullptr;
         return fail(WENOFD_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
     }
-    for (int k = 0; k < 3; ++k) L.graph_U[k] = v.U[k];
+    L.key = key;
     return 0;
 }
 
 }  // namespace
 
 namespace wenofd {
+
+GraphKey devloop_key(const DevLoop& L, const BatchView& v) {
+    GraphKey k;
+    k.add(v);
+    for (const void* p : {(const void*)L.tab_ghost, (const void*)L.tab_y, (const void*)L.tab_x}) k.add(p);
+    for (long long n : {(long long)L.n_ghost, (long long)L.n_y, (long long)L.n_x, (long long)L.tiles.x,
+                        (long long)L.tiles.y, (long long)L.scheme, (long long)L.rkmeth, (long long)L.fp,
+                        (long long)L.graph_steps, (long long)L.ghost_out})
+        k.add(n);
+    return k;
+}
 
 void devloop_drop_graph(DevLoop& L) {
     if (L.exec) (void)hipGraphExecDestroy(L.exec);

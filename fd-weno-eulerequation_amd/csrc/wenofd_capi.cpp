@@ -546,12 +546,32 @@ int enqueue_device_step(wenofd_solver* h, const BatchView& v, bool first) {
     return chk(launch_batch_advance(v, false, h->stream));
 }
 
+// Everything enqueue_device_step's launches take by value: the buffers (also
+// those outside the view: U, Ua, Ub, Fy, SPECIAL band, partials, time state,
+// descriptor, active count), the geometry, boundaries, coefficients and
+// kernel variants.
+GraphKey device_loop_key(const wenofd_solver* h, const BatchView& v) {
+    GraphKey k = devloop_key(h->loop, v);
+    k.add(h->g);
+    for (const void* p : {(const void*)h->U, (const void*)h->Ua, (const void*)h->Ub, (const void*)h->Fy,
+                          (const void*)h->special, (const void*)h->dt_part, (const void*)h->dt_part_nan,
+                          (const void*)h->loop_st_d})
+        k.add(p);
+    for (long long n : {(long long)h->bc.left, (long long)h->bc.right, (long long)h->bc.bottom,
+                        (long long)h->tiles.x, (long long)h->tiles.y, (long long)h->opt.scheme,
+                        (long long)h->opt.rkmeth, (long long)h->opt.fp_contract,
+                        (long long)(h->opt.testcase == kRTI ? 1 : 0), (long long)stage_ghosts(h).on})
+        k.add(n);
+    k.add(h->opt.cfl);
+    return k;
+}
+
 int enqueue_device_steps(wenofd_solver* h, const BatchView& v, int n) {
     DevLoop& L = h->loop;
     int rc;
     if (L.graph_steps > 0 && n >= L.graph_steps) {
-        if (L.exec && (L.graph_U[0] != h->U || L.graph_U[1] != h->Ua || L.graph_U[2] != h->Ub))
-            devloop_drop_graph(L);  // the graph holds the buffer addresses it was captured with
+        const GraphKey key = device_loop_key(h, v);
+        if (L.exec && !(L.key == key)) devloop_drop_graph(L);  // the graph holds what it was captured with
         if (!L.exec) {
             HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
             rc = 0;
@@ -569,9 +589,7 @@ int enqueue_device_steps(wenofd_solver* h, const BatchView& v, int n) {
                 L.exec = nullptr;
                 return fail(WENOFD_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
             }
-            L.graph_U[0] = h->U;
-            L.graph_U[1] = h->Ua;
-            L.graph_U[2] = h->Ub;
+            L.key = key;
         }
         for (; n >= L.graph_steps; n -= L.graph_steps) HIP_TRY(hipGraphLaunch(L.exec, h->stream));
     }
@@ -730,20 +748,40 @@ int wenofd_set_boundary(wenofd_solver* h) {
               : timed(h, WENOFD_K_GHOST, [&] { return launch_ghost_fill(h->U, h->special, h->g, h->bc, h->stream); });
 }
 
-int wenofd_get_flux(wenofd_solver* h) {
-    if (!h) return fail(WENOFD_E_ARG, "null handle");
+// The rhs of the current state into h->rhs: ghost fill, both sweeps and,
+// with source, the RTI gravity term fused into the x sweep.
+static int assemble(wenofd_solver* h, bool source) {
     int rc = set_dev(h);
     if (rc) return rc;
     if (!h->rhs) {
         HIP_TRY(hipMalloc(&h->rhs, h->state_doubles() * sizeof(double)));
         HIP_TRY(hipMemsetAsync(h->rhs, 0, h->state_doubles() * sizeof(double), h->stream));
     }
-    RkCoef k{kRkNone, h->opt.testcase == kRTI ? 1 : 0, 0.0, 0.0, 0.0};
+    RkCoef k{kRkNone, source && h->opt.testcase == kRTI ? 1 : 0, 0.0, 0.0, 0.0};
     if (h->sw_mode != WENOFD_SWITCH_OFF) {
         if ((rc = classify(h))) return rc;
         return stage(h, h->U, nullptr, nullptr, k, h->rhs, false, false);
     }
     return stage(h, h->U, nullptr, nullptr, k, h->rhs, false);
+}
+
+int wenofd_get_flux(wenofd_solver* h) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    return assemble(h, true);
+}
+
+int wenofd_assemble_boundary_term(wenofd_solver* h) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    return assemble(h, false);
+}
+
+int wenofd_assemble_source_term(wenofd_solver* h) {
+    if (!h) return fail(WENOFD_E_ARG, "null handle");
+    if (!h->rhs) return fail(WENOFD_E_ARG, "no rhs: call wenofd_assemble_boundary_term first");
+    int rc = set_dev(h);
+    if (rc || h->opt.testcase != kRTI) return rc;  // src/WENOFD.cpp:803-820: only RTI has a source
+    HIP_TRY(launch_add_source(h->U, h->rhs, h->g, h->stream));
+    return 0;
 }
 
 static int copy_out(wenofd_solver* h, const double* dev, double* host, int layout) {

@@ -20,6 +20,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "wenofd_index.hpp"
 #include "wenofd_kernels.hpp"
 #include "wenofd_math.hpp"
 #include "wenofd_switch.hpp"
@@ -41,20 +42,11 @@ constexpr bool kCapRecip = true;
 constexpr bool kCapRecip = false;
 #endif
 
-// Blocks are dispatched round-robin over the chip's 8 XCDs (block b runs on
-// XCD b mod 8), each with its own L2. xcd_block(b, n) maps b to a logical
-// block index so that every XCD works through one contiguous range of the n
-// logical blocks (a bijection on [0, n)): neighbouring tiles, which read the
-// same rows, then share one L2 (the x sweep: consecutive rows share a y-face
-// row; -0.7 % per step; the y sweep's tiles share only halo rows and gain
-// nothing).
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_block(int b, int n) {
-    const int per = (n + kXcds - 1) / kXcds;           // blocks of the XCDs that take the most
-    const int tall = n % kXcds == 0 ? kXcds : n % kXcds;  // how many XCDs take `per`; the rest per - 1
-    const int xcd = b % kXcds, local = b / kXcds;
-    return xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
-}
+// Blocks are ordered over the XCDs by xcd_block (wenofd_index.hpp): every
+// XCD works through one contiguous range of logical blocks, so neighbouring
+// tiles, which read the same rows, share one L2 (the x sweep: consecutive rows
+// share a y-face row; -0.7 % per step; the y sweep's tiles share only halo
+// rows and gain nothing).
 
 // ---------------------------------------------------------------------------
 // Ghost fill: exchangeGhostCellsValue (one-rank periodic-x ring) + setBoundary,
@@ -66,130 +58,45 @@ constexpr int kGhostThreads = 256;
 
 __device__ __forceinline__ void ghost_point(double* __restrict__ U, const double* __restrict__ special,
                                             const Geom& g, const Bc& bc, int k) {
-    const int nxg = 8 * g.ny;  // x-ghost points: 4 left + 4 right per interior row
-    const int nyg = 8 * g.nx;  // y-ghost points
-    if (k >= nxg + nyg) return;
-    const int sx = 4, ex = g.nx + 4, sy = 4, ey = g.ny + 4;
-    int di, dj, si, sj, flip = -1, spec = -1;
-    if (k < nxg) {
-        const int j = sy + k / 8;
-        const int e = k % 8;
-        dj = sj = j;
-        if (e < 4) {  // left ghost i = sx - e1
-            const int e1 = e + 1;
-            di = sx - e1;
-            if (bc.left == kBcPeriod) si = ex - 1 - e1;
-            else if (bc.left == kBcSpecial) { si = di; spec = 2 * g.nx + (j - sy); }
-            else { si = sx + e1; if (bc.left == kBcSlip) flip = 1; }
-        } else {  // right ghost i = ex - 1 + e1
-            const int e1 = e - 3;
-            di = ex - 1 + e1;
-            if (bc.right == kBcPeriod) si = sx + e1;
-            else if (bc.right == kBcSpecial) { si = di; spec = 2 * g.nx + g.ny + (j - sy); }
-            else { si = ex - 1 - e1; if (bc.right == kBcSlip) flip = 1; }
-        }
-    } else {
-        const int kk = k - nxg;
-        const int i = sx + kk / 8;
-        const int e = kk % 8;
-        di = si = i;
-        if (e < 4) {  // bottom ghost j = sy - e1 (the bottom boundary type governs both y sides, :1009)
-            const int e1 = e + 1;
-            dj = sy - e1;
-            if (bc.bottom == kBcPeriod) sj = ey - 1 - e1;
-            else if (bc.bottom == kBcSpecial) { sj = dj; spec = i - sx; }
-            else { sj = sy + e1; if (bc.bottom == kBcSlip) flip = 2; }
-        } else {  // top ghost j = ey - 1 + e1
-            const int e1 = e - 3;
-            dj = ey - 1 + e1;
-            if (bc.bottom == kBcPeriod) sj = sy + e1;
-            else if (bc.bottom == kBcSpecial) { sj = dj; spec = g.nx + (i - sx); }
-            else { sj = ey - 1 - e1; if (bc.bottom == kBcSlip) flip = 2; }
-        }
-    }
-    const size_t dst = (size_t)dj * g.pitch + di;
-    if (spec >= 0) {
+    if (k >= ghost_count(g)) return;
+    const GhostRef r = ghost_map(g, bc, k);
+    const size_t dst = (size_t)r.dj * g.pitch + r.di;
+    if (r.spec >= 0) {
         const int ns = 2 * (g.nx + g.ny);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) U[r * g.plane + dst] = special[r * ns + spec];
+        for (int q = 0; q < 4; ++q) U[q * g.plane + dst] = special[q * ns + r.spec];
         return;
     }
-    const size_t src = (size_t)sj * g.pitch + si;
+    const size_t src = (size_t)r.sj * g.pitch + r.si;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const double val = U[r * g.plane + src];
-        U[r * g.plane + dst] = (r == flip) ? -val : val;
+    for (int q = 0; q < 4; ++q) {
+        const double val = U[q * g.plane + src];
+        U[q * g.plane + dst] = (q == r.flip) ? -val : val;
     }
 }
 
-// ghost_point inverted: the ghost points whose source is interior point (i, j)
-// of U, holding v, get their values from it (copies, or the normal momentum
-// flipped at a slip side); a SPECIAL side's four ghosts next to (i, j) take
-// their fixed values. With nx, ny >= 5 every ghost of the fill has an interior
-// source, so calling this for every interior point leaves U ghost-filled
-// exactly as k_ghost_fill would. xs / ys (uniform over the caller's block)
-// enable the x-ghost part (points within 5 of the left or right edge) and the
-// y-ghost part (within 5 of the bottom or top).
+// ghost_point inverted (ghost_targets): the ghost points whose source is
+// interior point (i, j) of U, holding v, get their values from it (copies, or
+// the normal momentum flipped at a slip side); a SPECIAL side's four ghosts
+// next to (i, j) take their fixed values. With nx, ny >= 5 every ghost of the
+// fill has an interior source, so calling this for every interior point
+// leaves U ghost-filled exactly as k_ghost_fill would. xs / ys (uniform over
+// the caller's block) enable the x-ghost and the y-ghost parts.
 __device__ __forceinline__ void ghost_from_point(double* __restrict__ U, const Geom& g, const GhostOut& gh, int i,
                                                  int j, const double v[4], bool xs, bool ys) {
-    const int sx = 4, ex = g.nx + 4, sy = 4, ey = g.ny + 4;
-    auto put = [&](int di, int dj, int flip) {
-        const size_t d = (size_t)dj * g.pitch + di;
+    ghost_targets(
+        g, gh.bc, i, j, xs, ys,
+        [&](int di, int dj, int flip) {
+            const size_t d = (size_t)dj * g.pitch + di;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) U[r * g.plane + d] = (r == flip) ? -v[r] : v[r];
-    };
-    auto put_special = [&](int di, int dj, int spec) {
-        const size_t d = (size_t)dj * g.pitch + di;
-        const int ns = 2 * (g.nx + g.ny);
+            for (int r = 0; r < 4; ++r) U[r * g.plane + d] = (r == flip) ? -v[r] : v[r];
+        },
+        [&](int di, int dj, int spec) {
+            const size_t d = (size_t)dj * g.pitch + di;
+            const int ns = 2 * (g.nx + g.ny);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) U[r * g.plane + d] = gh.special[r * ns + spec];
-    };
-    // left ghosts i = sx - e1 and right ghosts i = ex - 1 + e1 of row j
-    if (!xs) {
-    } else if (gh.bc.left == kBcPeriod) {
-        const int e1 = ex - 1 - i;
-        if (e1 >= 1 && e1 <= 4) put(sx - e1, j, -1);
-    } else if (gh.bc.left == kBcSpecial) {
-        if (i == sx)
-#pragma unroll 1
-            for (int e1 = 1; e1 <= 4; ++e1) put_special(sx - e1, j, 2 * g.nx + (j - sy));
-    } else {
-        const int e1 = i - sx;
-        if (e1 >= 1 && e1 <= 4) put(sx - e1, j, gh.bc.left == kBcSlip ? 1 : -1);
-    }
-    if (!xs) {
-    } else if (gh.bc.right == kBcPeriod) {
-        const int e1 = i - sx;
-        if (e1 >= 1 && e1 <= 4) put(ex - 1 + e1, j, -1);
-    } else if (gh.bc.right == kBcSpecial) {
-        if (i == ex - 1)
-#pragma unroll 1
-            for (int e1 = 1; e1 <= 4; ++e1) put_special(ex - 1 + e1, j, 2 * g.nx + g.ny + (j - sy));
-    } else {
-        const int e1 = ex - 1 - i;
-        if (e1 >= 1 && e1 <= 4) put(ex - 1 + e1, j, gh.bc.right == kBcSlip ? 1 : -1);
-    }
-    // bottom ghosts j = sy - e1 and top ghosts j = ey - 1 + e1 of column i
-    if (!ys) {
-    } else if (gh.bc.bottom == kBcPeriod) {
-        const int e1 = ey - 1 - j;
-        if (e1 >= 1 && e1 <= 4) put(i, sy - e1, -1);
-        const int e2 = j - sy;
-        if (e2 >= 1 && e2 <= 4) put(i, ey - 1 + e2, -1);
-    } else if (gh.bc.bottom == kBcSpecial) {
-        if (j == sy)
-#pragma unroll 1
-            for (int e1 = 1; e1 <= 4; ++e1) put_special(i, sy - e1, i - sx);
-        if (j == ey - 1)
-#pragma unroll 1
-            for (int e1 = 1; e1 <= 4; ++e1) put_special(i, ey - 1 + e1, g.nx + (i - sx));
-    } else {
-        const int flip = gh.bc.bottom == kBcSlip ? 2 : -1;
-        const int e1 = j - sy;
-        if (e1 >= 1 && e1 <= 4) put(i, sy - e1, flip);
-        const int e2 = ey - 1 - j;
-        if (e2 >= 1 && e2 <= 4) put(i, ey - 1 + e2, flip);
-    }
+            for (int r = 0; r < 4; ++r) U[r * g.plane + d] = gh.special[r * ns + spec];
+        });
 }
 
 // st (optional): the solve's device-side time state; an inactive solve skips
@@ -209,11 +116,8 @@ __global__ __launch_bounds__(kGhostThreads) void k_ghost_fill(double* __restrict
 // themselves stay in a second ring of 8 rows for the faces' mean states. Loads
 // of U are coalesced along x; each point is read from HBM once per tile.
 // ---------------------------------------------------------------------------
-constexpr int kYC = 64;   // columns per tile
-constexpr int kYR = 4;    // face rows per step
-constexpr int kRing = 9;  // ring rows: a step reads 9 consecutive point rows
-// Point rows f-1 .. f+3 are read while rows f+2 .. f+5 are written: 8 slots.
-constexpr int kRawRing = 8;
+// (kYR face rows per step, the kRing-row split ring and the kRawRing-row point
+// ring: wenofd_index.hpp, where a host check replays their schedule.)
 
 // HYB: faces whose flag (flags[(f-4) nx + (i-4)]) is zero take the
 // component-wise upwind-5 flux; the flagged ones take the characteristic WENO
@@ -234,9 +138,9 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x / kYC);
     const int i = 4 + bx * kYC + tx;
     const bool col_ok = i < g.nx + 4;
-    const int F0 = 4 + by * YF;             // first face of the tile
-    const int F1 = min(F0 + YF, g.ny + 5);  // one past the last face
-    const int base = F0 - 3;                // point row held in ring slot 0
+    const int F0 = y_tile_first_face<YF>(by);   // first face of the tile
+    const int F1 = y_tile_end_face<YF>(g, by);  // one past the last face
+    const int base = F0 - 3;                    // point row held in ring slot 0
 
     auto load_row = [&](int prow) {
         const int slot = (prow - base) % kRing;
@@ -251,10 +155,10 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
                 sP[r][slot][tx] = fp[r];
                 sM[r][slot][tx] = fm[r];
             }
-            // Faces read the points from row F0-1 on. The lead-in's rows
-            // F0-3 and F0-2 are not stored: row F0-3 would share its slot
-            // with row F0+5, which the first step writes before any barrier.
-            if (prow - base >= 2) {
+            // not the lead-in's rows F0-3 and F0-2 (no face reads them, and
+            // row F0-3 shares its slot with row F0+5, which the first step
+            // writes before any barrier)
+            if (y_ring_keeps_point(F0, prow)) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) sU[r][(prow - base) % kRawRing][tx] = u[r];
             }
@@ -302,11 +206,6 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
         __syncthreads();
     }
 }
-
-template <int YF>
-__host__ __device__ inline int y_tiles_x(const Geom& g) { return (g.nx + kYC - 1) / kYC; }
-template <int YF>
-__host__ __device__ inline int y_tiles_y(const Geom& g) { return (g.ny + 1 + YF - 1) / YF; }
 
 // The WENO-ZP y sweep wants a few more registers than 3 waves per SIMD
 // allow; capping it there (a 3-dword spill) runs 2 % faster than 2 waves
@@ -389,10 +288,8 @@ __device__ __forceinline__ double local_dt_g(const double U[4], const Geom& g, d
 // tile also reduces the local CFL step of its updated cells (the next
 // calculateDeltaT) into *part / *part_nan.
 // ---------------------------------------------------------------------------
-// Cells per x tile of XT threads: XT - 5 faces read exactly XT points, so
-// every thread splits one point.
-template <int XT>
-constexpr int x_cells() { return XT - 6; }
+// Cells per x tile of XT threads (x_cells, wenofd_index.hpp): XT - 5 faces
+// read exactly XT points, so every thread splits one point.
 
 template <int S, int XT, bool HYB = false>
 __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, const double* __restrict__ Fy,
@@ -412,7 +309,7 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
     __shared__ int sNan[W];
     const int t = threadIdx.x;
     const int j = row + 4;
-    const int f0 = 4 + bx * XC;  // first face = first cell of the tile
+    const int f0 = x_tile_first_face<XT>(bx);  // first face = first cell of the tile
     const double* rowp = Uin + (size_t)j * g.pitch;
     // The tile's loads and split at raised wave priority, so that a starting
     // block issues its HBM requests ahead of the face work of the blocks
@@ -470,8 +367,8 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
     // ghost points of Uout to write: only the waves whose cells reach within 5
     // of the x edges, and rows within 5 of the y edges, have any (wave-uniform)
     const int wf = __builtin_amdgcn_readfirstlane(f0 + (t & ~63));  // the wave's first cell
-    const bool gh_x = k.gh.on && (wf <= 4 + 4 || wf + 63 >= g.nx + 4 - 5);
-    const bool gh_y = k.gh.on && (j <= 4 + 4 || j >= g.ny + 4 - 5);
+    const bool gh_x = k.gh.on && ghost_x_edge(g, wf, wf + 63);
+    const bool gh_y = k.gh.on && ghost_y_edge(g, j);
     const size_t o = (size_t)j * g.pitch + i;
     const size_t oy = (size_t)(j - 4) * g.fy_pitch + (i - 4);
     double fyb[4], fyt[4], unb[4];
@@ -534,9 +431,6 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
         }
     }
 }
-
-template <int XT>
-__host__ __device__ inline int x_tiles_x(const Geom& g) { return (g.nx + x_cells<XT>() - 1) / x_cells<XT>(); }
 
 // The contracted WENO-ZPI x sweep would take 170 VGPRs and run at 2 waves per
 // SIMD; capped at 3 it fits without spilling (-10 % sweep time), as does the
@@ -802,6 +696,16 @@ __global__ __launch_bounds__(XT) void k_batch_copy(BatchView b, int from, int to
     for (int r = 0; r < 4; ++r) b.U[to][o + r * d.g.plane] = b.U[from][o + r * d.g.plane];
 }
 
+// assembleSourceTerm, src/WENOFD.cpp:800-821 (RTI, g = 1): rhs_E += rho v and
+// rhs_{rho v} += rho at every interior point.
+__global__ __launch_bounds__(256) void k_add_source(const double* __restrict__ U, double* __restrict__ rhs, Geom g) {
+    const int i = 4 + blockIdx.x * 256 + threadIdx.x, j = 4 + blockIdx.y;
+    if (i >= g.nx + 4) return;
+    const size_t o = (size_t)j * g.pitch + i;
+    rhs[3 * g.plane + o] = rhs[3 * g.plane + o] + U[2 * g.plane + o];
+    rhs[2 * g.plane + o] = rhs[2 * g.plane + o] + U[o];
+}
+
 // Interior copy from -> to of one solve (forward Euler on the device loop).
 __global__ __launch_bounds__(256) void k_copy_interior(const double* __restrict__ from, double* __restrict__ to, Geom g,
                                                        const SolveState* __restrict__ st) {
@@ -914,6 +818,11 @@ hipError_t launch_ghost_fill(double* U, const double* special, const Geom& g, co
 
 hipError_t launch_copy_interior(const double* from, double* to, const Geom& g, const SolveState* st, hipStream_t s) {
     hipLaunchKernelGGL(k_copy_interior, dim3((g.nx + 255) / 256, g.ny), dim3(256), 0, s, from, to, g, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_source(const double* U, double* rhs, const Geom& g, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_source, dim3((g.nx + 255) / 256, g.ny), dim3(256), 0, s, U, rhs, g);
     return hipGetLastError();
 }
 

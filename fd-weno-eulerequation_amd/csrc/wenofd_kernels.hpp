@@ -121,6 +121,7 @@ namespace fused {
 using exact::tiles_valid;
 using exact::launch_ghost_fill;
 using exact::launch_copy_interior;
+using exact::launch_add_source;
 using exact::flux_x_rk_blocks;
 using exact::flux_y_blocks;
 using exact::ghost_blocks;

@@ -4,7 +4,9 @@
 //
 // Same method names and meanings as the reference: initializeSolver /
 // initializeAve / run / calculateDeltaT / RunRK1-3 / assembleRHS (= getFlux) /
-// setBoundary / outputAve / outputError / outputAccuracy, and the same
+// assembleBoundaryTerm / assembleSourceTerm / setBoundary /
+// exchangeGhostCellsValue / useWENO / outputAve / outputError /
+// outputAccuracy, the constructor from an EulerEquation pointer, and the same
 // config.cfg keys. Differences: the whole domain lives on one GPU (no MPI
 // slabs), and failures throw wenofd::Error instead of calling MPI_Abort.
 #pragma once
@@ -63,9 +65,17 @@ private:
     }
 };
 
+// Stands in for the reference's EulerEquation (utils/equations.hpp:55-699),
+// whose test-case set-up the library performs from the config keys: the
+// reference's main (src/main.cpp:13-26) passes one to the solver's
+// constructor, and the solver here only keeps the pointer.
+struct EulerEquation {};
+
 class CWENOFD {
 public:
     explicit CWENOFD(int device = -1) : m_device(device) {}
+    // include/WENOFD.hpp:34-37: the equation object is not owned.
+    explicit CWENOFD(EulerEquation* eq, int device = -1) : equation(eq), m_device(device) {}
     ~CWENOFD() {
         if (m_h) wenofd_destroy(m_h);
     }
@@ -101,9 +111,31 @@ public:
     void RunRK2(double deltaT) { check(wenofd_run_rk(m_h, 1, deltaT)); }
     void RunRK3(double deltaT) { check(wenofd_run_rk(m_h, 2, deltaT)); }
     // assembleRHS = ghost fill + getFlux (assembleBoundaryTerm) + source
+    // (src/WENOFD.cpp:454-465)
     void assembleRHS() { check(wenofd_get_flux(m_h)); }
     void getFlux() { assembleRHS(); }
+    // src/WENOFD.cpp:481-798: the flux terms of both directions (written into
+    // the rhs, which the reference's assembleRHS zeroes before calling it)
+    void assembleBoundaryTerm() { check(wenofd_assemble_boundary_term(m_h)); }
+    // src/WENOFD.cpp:800-821: adds the RTI gravity source to the rhs
+    void assembleSourceTerm() { check(wenofd_assemble_source_term(m_h)); }
+    // src/WENOFD.cpp:998-1159
     void setBoundary() { check(wenofd_set_boundary(m_h)); }
+    // src/WENOFD.cpp:850-929. With the whole domain on one device the halo
+    // ring reduces to its self-send, which fills the periodic x ghosts: that
+    // fill is part of the library's ghost fill, so this refills every ghost
+    // point (a superset of what the reference's call writes, from the same
+    // state).
+    void exchangeGhostCellsValue() { check(wenofd_set_boundary(m_h)); }
+    // src/WENOFD.cpp:823-848: the scalar WENO5 reconstruction of the solve's
+    // scheme at j+1/2 from (u_{j-2}, ..., u_{j+2}), run by the device
+    // arithmetic the flux kernels use (WENO-ZP with lambda = dx^(2/3)).
+    double useWENO(double& uavemm, double& uavem, double& uave, double& uavep, double& uavepp) {
+        const double v[5] = {uavemm, uavem, uave, uavep, uavepp};
+        double out = 0;
+        check(wenofd_probe_weno5(m_info.device, m_info.scheme, m_info.zp_lambda, 1, v, &out));
+        return out;
+    }
     void outputAve(const std::string& prefix) { check(wenofd_output_ave(m_h, prefix.c_str())); }
     void outputError(const std::string& prefix) { check(wenofd_output_error(m_h, prefix.c_str())); }
     void outputAccuracy(const std::string& prefix) { check(wenofd_output_accuracy(m_h, prefix.c_str(), nullptr)); }
@@ -139,6 +171,7 @@ public:
     wenofd_solver* handle() const { return m_h; }
 
     // Public state mirroring the reference's members (refreshed after calls).
+    EulerEquation* equation = nullptr;
     double m_now = 0, m_deltaX = 0, m_deltaY = 0, m_cfl = 0, m_outputTime = 0;
     int m_worldPointNumX = 0, m_worldPointNumY = 0;
     int m_ghostCellNum = 4;

@@ -20,12 +20,16 @@ KERNEL_NAMES = ("ghost_fill", "flux_x_rk", "flux_y", "switch", "delta_t")
 SWITCH_OFF, SWITCH_MLP, SWITCH_LINEAR, SWITCH_WENO = range(4)
 N_PARAMS = 1441
 SWITCH_WEIGHTS = os.path.join(HERE, "switch", "switch_mlp.npz")
+# Steps per graph replay given to every new Solver / Batch (None: the
+# library's default of 16; 0 launches every step from the host).
+DEFAULT_GRAPH_STEPS = None
 
 # Exported symbols declared in include/wenofd_amd.h.
 EXPORTS = (
     "wenofd_last_error", "wenofd_version", "wenofd_create", "wenofd_create_from_file", "wenofd_destroy",
     "wenofd_parse_config", "wenofd_initialize_ave", "wenofd_calculate_delta_t", "wenofd_run_rk",
-    "wenofd_set_boundary", "wenofd_get_flux", "wenofd_get_rhs", "wenofd_step", "wenofd_advance", "wenofd_run",
+    "wenofd_set_boundary", "wenofd_get_flux", "wenofd_get_rhs", "wenofd_assemble_boundary_term",
+    "wenofd_assemble_source_term", "wenofd_step", "wenofd_advance", "wenofd_run",
     "wenofd_get_state", "wenofd_set_state", "wenofd_get_info", "wenofd_set_time", "wenofd_output_dir",
     "wenofd_output_ave", "wenofd_output_error", "wenofd_output_accuracy", "wenofd_accuracy",
     "wenofd_set_timing", "wenofd_get_timing", "wenofd_synchronize", "wenofd_probe_weno5",
@@ -72,7 +76,8 @@ def lib():
         L.wenofd_create_from_file.argtypes = [ctypes.c_char_p, i, ctypes.POINTER(vp)]
         L.wenofd_destroy.argtypes = [vp]
         L.wenofd_parse_config.argtypes = [ctypes.c_char_p, ctypes.POINTER(Info)]
-        for n in ("wenofd_initialize_ave", "wenofd_set_boundary", "wenofd_get_flux", "wenofd_synchronize"):
+        for n in ("wenofd_initialize_ave", "wenofd_set_boundary", "wenofd_get_flux", "wenofd_synchronize",
+                  "wenofd_assemble_boundary_term", "wenofd_assemble_source_term"):
             getattr(L, n).argtypes = [vp]
         L.wenofd_calculate_delta_t.argtypes = [vp, _dp]
         L.wenofd_run_rk.argtypes = [vp, i, d]
@@ -206,6 +211,8 @@ class Solver:
         self.h = h
         inf = self.info()
         self.nx, self.ny = inf.nx, inf.ny
+        if DEFAULT_GRAPH_STEPS is not None:
+            self.set_device_loop(DEFAULT_GRAPH_STEPS)
 
     def close(self):
         if getattr(self, "h", None):
@@ -253,6 +260,12 @@ class Solver:
 
     def get_flux(self):
         _check(lib().wenofd_get_flux(self.h))
+
+    def assemble_boundary_term(self):
+        _check(lib().wenofd_assemble_boundary_term(self.h))
+
+    def assemble_source_term(self):
+        _check(lib().wenofd_assemble_source_term(self.h))
 
     def get_rhs(self, layout=LAYOUT_REF):
         out = np.empty(self.shape(layout))
@@ -475,6 +488,8 @@ class Batch:
         self.h = h
         self.n = len(texts)
         self.shapes = [(inf.nx, inf.ny) for inf in (self.info(s) for s in range(self.n))]
+        if DEFAULT_GRAPH_STEPS is not None:
+            self.set_graph_steps(DEFAULT_GRAPH_STEPS)
 
     def close(self):
         if getattr(self, "h", None):

@@ -124,6 +124,13 @@ int wenofd_set_boundary(wenofd_solver* h);
  * the result is read with wenofd_get_rhs. */
 int wenofd_get_flux(wenofd_solver* h);
 int wenofd_get_rhs(wenofd_solver* h, double* host, int layout);
+/* CWENOFD::assembleBoundaryTerm (src/WENOFD.cpp:481-798): ghost fill and the
+ * flux terms of both directions into the rhs (which the reference's only
+ * caller, assembleRHS, has zeroed first: the rhs is written, not added to). */
+int wenofd_assemble_boundary_term(wenofd_solver* h);
+/* CWENOFD::assembleSourceTerm (src/WENOFD.cpp:800-821): adds the RTI gravity
+ * source of the current state to the rhs (no other case has one). */
+int wenofd_assemble_source_term(wenofd_solver* h);
 
 /* One iteration of the run() loop (src/WENOFD.cpp:213-239): dt =
  * calculateDeltaT (or dt_fixed if > 0), RunRK<RKMETH>(dt), now += dt. */

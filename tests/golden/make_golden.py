@@ -213,6 +213,26 @@ def gen_run(tmp, out, big):
     np.savez_compressed(os.path.join(out, "run_accuracy.npz"), **res)
 
 
+def gen_run_files(tmp, out):
+    """The reference's own CWENOFD::run on SMOOTH 16^2 (WENO-JS and WENO-Z, RK3,
+    CFL 0.5): its Tecplot solution (initial, final) and error files and its
+    accuracy CSV, kept verbatim as the expected output files, from the build
+    without contraction (the one the GPU path equals bit for bit) and from the
+    makefile build."""
+    for b in BUILDS:
+        for sc in (0, 1):
+            d = tempfile.mkdtemp(dir=tmp)
+            harness(b, ["run", 0, sc, 2, f17(0.5), 16, 16], d)
+            src = glob.glob(os.path.join(d, "output", "*_SMOOTH_*_16x16"))[0]
+            dst = os.path.join(out, "run_files", f"{b}_smooth_s{sc}_n16_cfl0.5")
+            os.makedirs(dst, exist_ok=True)
+            for name in ("solution_initial.plt", "solution_final.plt", "error_final.plt", "accuracy_final.csv"):
+                with open(os.path.join(src, name), "rb") as f:
+                    data = f.read()
+                with open(os.path.join(dst, name), "wb") as f:
+                    f.write(data)
+
+
 def gen_c2(tmp, out):
     """C2: four-quadrant 800^2, WENO-Z, RK3, step-0 CFL-0.7 dt, 200 steps."""
     spec = (5, 1, 2, 0.7, 800, 800, 200, 1, -1, [])
@@ -259,7 +279,7 @@ def main():
     tmp = tempfile.mkdtemp(prefix="wenofd_golden_")
     jobs = {"weno": lambda: gen_weno(tmp, HERE), "physics": lambda: gen_physics(tmp, HERE),
             "rhs": lambda: gen_rhs(tmp, HERE), "traj": lambda: gen_traj(tmp, HERE),
-            "run": lambda: gen_run(tmp, HERE, not args.skip_big)}
+            "run": lambda: gen_run(tmp, HERE, not args.skip_big), "run_files": lambda: gen_run_files(tmp, HERE)}
     if not args.skip_big:
         jobs["c2"] = lambda: gen_c2(tmp, HERE)
         jobs["vortex"] = lambda: gen_vortex(tmp, HERE)

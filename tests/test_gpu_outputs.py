@@ -60,3 +60,28 @@ def test_dropin_program_end_to_end(tmp_path, golden):
     assert len(csv) == 1
     ref = golden("run_accuracy")["smooth_s1_n16_cfl0.5"]
     assert np.allclose(_read_accuracy(csv[0])[1:], ref[1:], rtol=1e-11, atol=0)
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_run_files_equal_the_references(tmp_path, golden, scheme):
+    """CWENOFD::run on SMOOTH 16^2 (RK3, CFL 0.5) writes the same four files as
+    the reference's own run() (src/WENOFD.cpp:1161-1382): the Tecplot solution
+    (initial and final) and error files and the accuracy CSV, byte for byte
+    against the reference built without contraction (whose state the GPU path
+    equals bit for bit), and the solution files also against its makefile
+    build."""
+    print(f"[gpu] run() SMOOTH 16x16 scheme {scheme} RK3 CFL 0.5: files against the reference's", flush=True)
+    with W.Solver(TESTCASE=0, SCHEME=scheme, RKMETH=2, CFL=0.5, ELEMNUMX=16, ELEMNUMY=16,
+                  OUTPUTDIR=str(tmp_path) + "/") as s:
+        s.run()
+        d = s.output_dir()
+    ref = os.path.join(ROOT, "tests", "golden", "run_files")
+    for build, names in (("ref_nofma", ("solution_initial.plt", "solution_final.plt", "error_final.plt",
+                                        "accuracy_final.csv")),
+                         ("ref_native", ("solution_initial.plt", "solution_final.plt"))):
+        for name in names:
+            with open(os.path.join(d, name), "rb") as f:
+                got = f.read()
+            with open(os.path.join(ref, f"{build}_smooth_s{scheme}_n16_cfl0.5", name), "rb") as f:
+                want = f.read()
+            assert got == want, (build, name)

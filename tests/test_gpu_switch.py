@@ -315,6 +315,41 @@ def test_training_gradient_matches_oracle():
     g.close()
 
 
+def test_adam_steps_follow_kingma_ba():
+    """k_adam is Adam (Kingma & Ba 2015, Algorithm 1) with lr, beta1 0.9,
+    beta2 0.999, eps 1e-8. From zero moments its first step is, per
+    parameter, -lr g / (|g| + eps): lr against the sign of the gradient
+    wherever |g| >> eps. The second step is the published update recomputed
+    in float64 from the two full-batch gradients (each taken at the
+    parameters the step starts from)."""
+    rng = np.random.default_rng(11)
+    n, lr = 512, 1e-3
+    X = rng.uniform(-1, 1, (n, 10)).astype(np.float32)
+    Y = (X[:, 2] - X[:, 7] > 0.1).astype(np.uint8)
+    idx = np.arange(n, dtype=np.int32)
+    with W.Mlp(seed=5) as m:
+        m.add(X, Y)
+        P0 = m.params.astype(np.float64)
+        _, g1 = m.grad(idx)
+        m.train(epochs=1, batch=n, lr=lr, seed=1)
+        P1 = m.params.astype(np.float64)
+        _, g2 = m.grad(idx)
+        m.train(epochs=1, batch=n, lr=lr, seed=2)
+        P2 = m.params.astype(np.float64)
+    g1, g2 = g1.astype(np.float64), g2.astype(np.float64)
+    big = np.abs(g1) > 1e-4
+    assert big.sum() > 1000
+    step1 = P1 - P0
+    assert np.all(np.abs(step1) <= lr * (1 + 1e-5))
+    assert np.all(np.abs(step1[big] + lr * np.sign(g1[big])) <= 2e-3 * lr)
+    m2 = 0.9 * (0.1 * g1) + 0.1 * g2
+    v2 = 0.999 * (0.001 * g1 * g1) + 0.001 * g2 * g2
+    c1, c2 = 1 - 0.9 ** 2, 1 - 0.999 ** 2
+    want = -lr * (m2 / c1) / (np.sqrt(v2 / c2) + 1e-8)
+    ok = big & (np.abs(g2) > 1e-4)
+    assert np.all(np.abs((P2 - P1)[ok] - want[ok]) <= 5e-3 * lr)
+
+
 def test_harvest_keeps_a_deterministic_subset():
     g, o = pair(5, 30, 26, seed=21)
     Xall = np.concatenate([g.harvest(0)[0].reshape(-1, 10), g.harvest(1)[0].reshape(-1, 10)])

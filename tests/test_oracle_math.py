@@ -217,3 +217,48 @@ def test_op_count_is_positive_and_scheme_dependent():
     z = O.count_ops(0, 1, 2, 12, 12)
     assert js["div"] > 0 and js["sqrt"] > 0
     assert z["add"] != js["add"]
+
+
+def test_seeded_quadrant_follows_published_splitmix64():
+    """The seeded four-quadrant states come from splitmix64 (Vigna), whose
+    published first outputs for seed 0 are fixed below; each output x gives
+    the uniform (x >> 11) 2^-53, drawn per quadrant (UR, UL, LL, LR) in the
+    order rho ~ U[0.1, 2], u, v ~ U[-1.5, 1.5], p ~ U[0.05, 2]."""
+    published = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F, 0xF88BB8A8724C81EC,
+                 0x1B39896A51A8749B]
+    u = [(x >> 11) * (1.0 / 9007199254740992.0) for x in published]
+    q = O.seeded_quadrant(0)
+    assert q[0, 0] == 0.1 + 1.9 * u[0]
+    assert q[0, 1] == -1.5 + 3.0 * u[1]
+    assert q[0, 2] == -1.5 + 3.0 * u[2]
+    assert q[0, 3] == 0.05 + 1.95 * u[3]
+    assert q[1, 0] == 0.1 + 1.9 * u[4]
+
+
+def test_isentropic_vortex_is_a_steady_vortex():
+    """The vortex initial condition (Shu's isentropic vortex, strength 5 at
+    (5, 5) on top of (rho, u, v, p) = (1, 1, 1, 1), gamma 1.4) is an exact
+    solution translated with the background flow iff, in the frame moving
+    with it, the pressure gradient balances the centripetal acceleration,
+    dp/dr = rho v_theta^2 / r, and the flow is isentropic, p = rho^gamma.
+    Checked on the oracle's initial state along the ray y = 5, x > 5 by
+    fourth-order central differences (1e-5 relative at this spacing); a
+    wrong factor in the velocity or temperature perturbation breaks the
+    balance by O(1)."""
+    n, gamma = 401, 1.4
+    U = O.Solver(4, 1, 2, 0.05, n, n).get_state()
+    _, xc, yc = O.Solver(4, 1, 2, 0.05, n, n).grid()
+    x, y = xc[4:-4], yc[4:-4]
+    j = int(np.argmin(np.abs(y - 5.0)))
+    assert abs(y[j] - 5.0) < 1e-12
+    rho = U[:, j, 0]
+    u, v = U[:, j, 1] / rho, U[:, j, 2] / rho
+    p = (gamma - 1) * (U[:, j, 3] - 0.5 * rho * (u * u + v * v))
+    assert np.allclose(p, rho ** gamma, rtol=1e-12, atol=0)
+    assert np.allclose(u, 1.0, rtol=0, atol=1e-12)  # no radial velocity on the ray
+    h = x[1] - x[0]
+    for i in range(np.searchsorted(x, 5.3), np.searchsorted(x, 8.0)):
+        r = x[i] - 5.0
+        vt = v[i] - 1.0  # tangential velocity in the vortex frame
+        dpdr = (-p[i + 2] + 8 * p[i + 1] - 8 * p[i - 1] + p[i - 2]) / (12 * h)
+        assert abs(dpdr - rho[i] * vt * vt / r) <= 1e-4 * abs(dpdr) + 1e-12, (r, dpdr, rho[i] * vt * vt / r)
