@@ -33,7 +33,7 @@ EXPORTS = (
     "wenofd_get_state", "wenofd_set_state", "wenofd_get_info", "wenofd_set_time", "wenofd_output_dir",
     "wenofd_output_ave", "wenofd_output_error", "wenofd_output_accuracy", "wenofd_accuracy",
     "wenofd_set_timing", "wenofd_get_timing", "wenofd_synchronize", "wenofd_probe_weno5",
-    "wenofd_probe_physics", "wenofd_set_tiles", "wenofd_set_device_loop",
+    "wenofd_probe_physics", "wenofd_set_tiles", "wenofd_set_device_loop", "wenofd_delta_t_noalloc",
     "wenofd_batch_create", "wenofd_batch_destroy", "wenofd_batch_size", "wenofd_batch_initialize_ave",
     "wenofd_batch_advance", "wenofd_batch_set_graph_steps", "wenofd_batch_set_tiles", "wenofd_batch_get_tiles",
     "wenofd_batch_autotune", "wenofd_batch_get_state", "wenofd_batch_get_info", "wenofd_batch_accuracy",
@@ -100,6 +100,7 @@ def lib():
         L.wenofd_probe_physics.argtypes = [i, d, i, _dp, _dp]
         L.wenofd_set_tiles.argtypes = [vp, i, i]
         L.wenofd_set_device_loop.argtypes = [vp, i]
+        L.wenofd_delta_t_noalloc.argtypes = [vp, i, i, i, d, d, d, d, d, i, vp, vp]
         ip = ctypes.POINTER(i)
         L.wenofd_batch_create.argtypes = [ctypes.POINTER(ctypes.c_char_p), i, i, ctypes.POINTER(vp)]
         for n in ("wenofd_batch_destroy", "wenofd_batch_initialize_ave", "wenofd_batch_synchronize"):
@@ -179,6 +180,13 @@ def probe_physics(states, gamma=1.4, device=0):
     out = np.empty((s.shape[0], 82))
     _check(lib().wenofd_probe_physics(device, float(gamma), s.shape[0], _ptr(s), _ptr(out)))
     return out
+
+
+def delta_t_noalloc(U_ptr, nx, ny, pitch, gamma, dx, dy, cfl, t_left, fp_contract, out_ptr, stream=None):
+    """wenofd_delta_t_noalloc on device pointers (ints): {dt, nan} into out_ptr."""
+    _check(lib().wenofd_delta_t_noalloc(ctypes.c_void_p(U_ptr), int(nx), int(ny), int(pitch), float(gamma),
+                                        float(dx), float(dy), float(cfl), float(t_left), int(fp_contract),
+                                        ctypes.c_void_p(out_ptr), ctypes.c_void_p(stream)))
 
 
 def vortex_batch_configs(n=64, seed=0, scheme=1, rkmeth=2, cfl=0.05, sizes=range(100, 401, 20), t_end=None):

@@ -165,6 +165,28 @@ int wenofd_output_accuracy(wenofd_solver* h, const char* prefix, double* norms3)
  * without writing a file (norms3 = {Linf, L1, L2}). */
 int wenofd_accuracy(wenofd_solver* h, double* norms3);
 
+/*
+ * calculateDeltaT of a caller-owned device state, allocating nothing. No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops). Same kernels
+ * and result as wenofd_calculate_delta_t (src/WENOFD.cpp:398-452), written to
+ * device memory instead of returned to the host:
+ *   U      device state in the library's layout, [4][ny + 8][pitch] doubles
+ *          (rho, rho u, rho v, E; interior points i, j in [4, n + 4), x
+ *          fastest); only interior points are read;
+ *   out    device array of 2 doubles: {dt, nan}, dt = the minimum of 1.0 and
+ *          cfl / ((|u| + c) / dx + (|v| + c) / dy) over the interior, clipped
+ *          to t_left (t_end - now); nan = 1.0 when any point gives a NaN;
+ *   stream a hipStream_t (null: the null stream); the call only enqueues.
+ * nx, ny in [5, 65536], pitch >= nx + 8; fp_contract as the FP_CONTRACT key.
+ * Scratch: a file-scope device array of 1024 partial minima (12 KiB per GPU,
+ * held until the process ends) instead of hipMalloc'ed buffers; every call on
+ * one GPU shares it, so calls that overlap on one GPU race: serialise them
+ * (one stream, or events).
+ */
+int wenofd_delta_t_noalloc(const double* U, int nx, int ny, int pitch, double gamma, double dx, double dy,
+                           double cfl, double t_left, int fp_contract, double* out, void* stream);
+
 /* Per-kernel timing with HIP events on the solver's stream. While enabled,
  * each launch is bracketed by events; wenofd_get_timing synchronises and
  * returns the summed milliseconds and launch counts per WENOFD_K_* kind, then
