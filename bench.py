@@ -278,7 +278,9 @@ def hybrid(W, n, seed, t_end, truth, fp):
              "rho": s.get_state()[..., 0]}
         if interval:
             fx, fy = s.switch_flags()
-            r["flag_rate_final_dilated"] = float((fx.sum() + fy.sum()) / (fx.size + fy.size))
+            m = np.concatenate([fx.ravel(), fy.ravel()])
+            r["flag_rate_final_dilated"] = float((m != 0).mean())  # faces with any field flagged
+            r["field_rate_final_dilated"] = float(np.unpackbits(m[:, None], axis=1)[:, 4:].mean())
             r["fallbacks"] = s.switch_fallbacks()
         s.close()
         return r
@@ -307,7 +309,8 @@ def hybrid(W, n, seed, t_end, truth, fp):
         d = np.abs(h["rho"] - plain["rho"])
         c = {"interval": interval, "dilation": radius, "value": h["value"], "ms_per_step": h["ms_per_step"],
              "speedup": h["value"] / plain["value"], "steps": h["steps"],
-             "flag_rate_final_dilated": h["flag_rate_final_dilated"], "fallbacks": h["fallbacks"],
+             "flag_rate_final_dilated": h["flag_rate_final_dilated"],
+             "field_rate_final_dilated": h["field_rate_final_dilated"], "fallbacks": h["fallbacks"],
              "rho_delta_vs_plain": {"L1": float(d.mean()), "Linf": float(d.max())}}
         if rt is not None:
             eh = np.abs(h["rho"] - rt)

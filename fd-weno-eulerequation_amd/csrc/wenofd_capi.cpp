@@ -94,6 +94,20 @@ struct wenofd_solver {
     unsigned char* fl_raw_y = nullptr;
     unsigned char* fl_x = nullptr;
     unsigned char* fl_y = nullptr;
+    // The faces of the dilated masks that flag any field (rebuilt whenever the
+    // masks change): the list (grouped by flagged-field count), each face's
+    // slot (its list position, ~0 for none), the faces' summed corrections (4
+    // doubles each, per stage), the face counts on the device and the list
+    // build's scratch.
+    unsigned* ent_x = nullptr;
+    unsigned* ent_y = nullptr;
+    unsigned* slot_x = nullptr;
+    unsigned* slot_y = nullptr;
+    double* corr_x = nullptr;
+    double* corr_y = nullptr;
+    int* n_pairs = nullptr;  // {x, y}
+    int* list_cnt = nullptr;
+    int* list_off = nullptr;
     // After a switched step the state before it survives in *prev_slot (one of
     // Ua / Ub) until the next step; the plain solver redoes the step from it
     // when the switched step produced a NaN time step.
@@ -172,7 +186,8 @@ int set_dev(wenofd_solver* h) {
 
 // One RK stage: ghost fill of `in` (unless done), y-face fluxes, then x faces
 // fused with the rhs and the update into `out` (which also reduces the next
-// CFL step when last_stage is set). With the switch on, faces use the flags.
+// CFL step when last_stage is set). With the switch on, one hybrid stage
+// kernel does all of it with the field masks fl_x / fl_y.
 int stage(wenofd_solver* h, double* in, const double* un, double* out, const RkCoef& k, double* rhs_out,
           bool last_stage, bool ghost = true) {
     int rc;
@@ -180,30 +195,53 @@ int stage(wenofd_solver* h, double* in, const double* un, double* out, const RkC
     if (ghost &&
         (rc = timed(h, WENOFD_K_GHOST, [&] { return launch_ghost_fill(in, h->special, h->g, h->bc, h->stream); })))
         return rc;
+    if (sw) {
+        const int fp = h->opt.fp_contract;
+        if ((rc = timed(h, WENOFD_K_PATCH, [&] {
+                 return WENOFD_FP(fp, launch_hyb_patch(h->opt.scheme, in, h->g, h->ent_x, h->ent_y, h->fl_x,
+                                                       h->fl_y, h->n_pairs, h->corr_x, h->corr_y, h->stream));
+             })))
+            return rc;
+        if ((rc = timed(h, WENOFD_K_HYB, [&] {
+                 return WENOFD_FP(fp, launch_hyb_stage(in, un, out, rhs_out, h->g, k, h->opt.cfl,
+                                                       last_stage ? h->dt_part : nullptr,
+                                                       last_stage ? h->dt_part_nan : nullptr, h->slot_x, h->slot_y,
+                                                       h->corr_x, h->corr_y, h->stream));
+             })))
+            return rc;
+        if (last_stage) h->dt_parts = (int)hyb_stage_blocks(h->g);
+        return 0;
+    }
     if ((rc = timed(h, WENOFD_K_FLUX_Y, [&] {
-             return WENOFD_FP(h->opt.fp_contract,
-                              launch_flux_y(h->opt.scheme, h->tiles, in, h->Fy, h->g, h->stream, sw ? h->fl_y : nullptr));
+             return WENOFD_FP(h->opt.fp_contract, launch_flux_y(h->opt.scheme, h->tiles, in, h->Fy, h->g, h->stream));
          })))
         return rc;
     if ((rc = timed(h, WENOFD_K_FLUX_X, [&] {
              return WENOFD_FP(h->opt.fp_contract,
                               launch_flux_x_rk(h->opt.scheme, h->tiles, in, h->Fy, un, out, rhs_out, h->g, k,
                                                h->opt.cfl, last_stage ? h->dt_part : nullptr,
-                                               last_stage ? h->dt_part_nan : nullptr, h->stream,
-                                               sw ? h->fl_x : nullptr));
+                                               last_stage ? h->dt_part_nan : nullptr, h->stream));
          })))
         return rc;
     if (last_stage) h->dt_parts = (int)flux_x_rk_blocks(h->g, h->tiles);
     return 0;
 }
 
-// Ghost fill of U, then (MLP mode) the flags of every face from the network,
-// dilated by sw_every faces in both directions of the face grid: the flags are
-// computed from the initial state of every sw_every-th step and serve all
-// stages of that many steps (a wave moves less than one cell per step at
-// CFL <= 1). The y faces' flags are then shared by each aligned group of 64
-// columns of a face row, the faces one wave of the y sweep computes: a wave
-// with one flagged face pays the WENO flux anyway, and so skips the cheap one.
+// Ghost fill of U, then (MLP mode) the field masks of every face from the
+// network, each field's flags dilated by the radius (default sw_every faces)
+// in both directions of the face grid: the masks are computed from the
+// initial state of every sw_every-th step and serve all stages of that many
+// steps (a wave moves less than one cell per step at CFL <= 1).
+// The pair lists and slots of the current dilated masks fl_x / fl_y.
+hipError_t build_lists(wenofd_solver* h) {
+    hipError_t e = launch_build_list(h->fl_x, h->faces_x(), h->list_cnt, h->list_off, h->ent_x, h->slot_x,
+                                     h->n_pairs, h->stream);
+    if (e == hipSuccess)
+        e = launch_build_list(h->fl_y, h->faces_y(), h->list_cnt, h->list_off, h->ent_y, h->slot_y, h->n_pairs + 1,
+                              h->stream);
+    return e;
+}
+
 int classify(wenofd_solver* h, float* logits_x = nullptr, float* logits_y = nullptr, bool dilate = true,
              bool force = true) {
     int rc;
@@ -218,17 +256,31 @@ int classify(wenofd_solver* h, float* logits_x = nullptr, float* logits_y = null
         if (e == hipSuccess)
             e = launch_classify(1, h->U, h->g, h->sw_params, h->sw_thr, h->fl_raw_y, logits_y, h->stream);
         if (e == hipSuccess && dilate)
-            e = launch_dilate(h->fl_raw_x, h->fl_x, h->g.ny, h->g.nx + 1, rad, false, h->stream);
+            e = launch_dilate(h->fl_raw_x, h->fl_x, h->g.ny, h->g.nx + 1, rad, h->stream);
         if (e == hipSuccess && dilate)
-            e = launch_dilate(h->fl_raw_y, h->fl_y, h->g.ny + 1, h->g.nx, rad, true, h->stream);
+            e = launch_dilate(h->fl_raw_y, h->fl_y, h->g.ny + 1, h->g.nx, rad, h->stream);
+        if (e == hipSuccess && dilate) e = build_lists(h);
         return e;
     });
 }
 
-// RK methods with the switch on: flags from the step's initial state, and the
-// final stage written outside U so that the initial state survives the step.
+// The ghost points an RK stage's x sweep writes into its output (RK2 / RK3):
+// the next stage then needs no ghost fill. Off for grids under 5 points a side.
+GhostOut stage_ghosts(const wenofd_solver* h) {
+    GhostOut gh;
+    gh.on = h->g.nx >= 5 && h->g.ny >= 5;
+    gh.bc = h->bc;
+    gh.special = h->special;
+    return gh;
+}
+
+// RK methods with the switch on: masks from the step's initial state (or an
+// earlier one, every sw_every steps), and the final stage written outside U so
+// that the initial state survives the step. With RK2 / RK3 every stage writes
+// its output's ghost points, so only the classification's fill of U runs.
 int run_rk_switched(wenofd_solver* h, int rkmeth, double dt) {
     const int grav = h->opt.testcase == kRTI ? 1 : 0;
+    const GhostOut gh = stage_ghosts(h);
     int rc;
     h->dt_parts = 0;
     h->prev_slot = nullptr;
@@ -240,19 +292,19 @@ int run_rk_switched(wenofd_solver* h, int rkmeth, double dt) {
         std::swap(h->U, h->Ua);
         h->prev_slot = &h->Ua;
     } else if (rkmeth == 1) {
-        RkCoef k1{kRkStage, grav, 1.0, 0.0, 1.0 * dt};
-        RkCoef k2{kRkStage, grav, 0.5, 0.5, 0.5 * dt};
+        RkCoef k1{kRkStage, grav, 1.0, 0.0, 1.0 * dt, gh};
+        RkCoef k2{kRkStage, grav, 0.5, 0.5, 0.5 * dt, gh};
         if ((rc = stage(h, h->U, h->U, h->Ua, k1, nullptr, false, false))) return rc;
-        if ((rc = stage(h, h->Ua, h->U, h->Ub, k2, nullptr, true))) return rc;
+        if ((rc = stage(h, h->Ua, h->U, h->Ub, k2, nullptr, true, !gh.on))) return rc;
         std::swap(h->U, h->Ub);
         h->prev_slot = &h->Ub;
     } else if (rkmeth == 2) {
-        RkCoef k1{kRkStage, grav, 1.0, 0.0, 1.0 * dt};
-        RkCoef k2{kRkStage, grav, 3.0 / 4.0, 1.0 / 4.0, (1.0 / 4.0) * dt};
-        RkCoef k3{kRkStage, grav, 1.0 / 3.0, 2.0 / 3.0, (2.0 / 3.0) * dt};
+        RkCoef k1{kRkStage, grav, 1.0, 0.0, 1.0 * dt, gh};
+        RkCoef k2{kRkStage, grav, 3.0 / 4.0, 1.0 / 4.0, (1.0 / 4.0) * dt, gh};
+        RkCoef k3{kRkStage, grav, 1.0 / 3.0, 2.0 / 3.0, (2.0 / 3.0) * dt, gh};
         if ((rc = stage(h, h->U, h->U, h->Ua, k1, nullptr, false, false))) return rc;
-        if ((rc = stage(h, h->Ua, h->U, h->Ub, k2, nullptr, false))) return rc;
-        if ((rc = stage(h, h->Ub, h->U, h->Ua, k3, nullptr, true))) return rc;
+        if ((rc = stage(h, h->Ua, h->U, h->Ub, k2, nullptr, false, !gh.on))) return rc;
+        if ((rc = stage(h, h->Ub, h->U, h->Ua, k3, nullptr, true, !gh.on))) return rc;
         std::swap(h->U, h->Ua);
         h->prev_slot = &h->Ua;
     } else {
@@ -260,16 +312,6 @@ int run_rk_switched(wenofd_solver* h, int rkmeth, double dt) {
     }
     h->prev_dt = dt;
     return 0;
-}
-
-// The ghost points an RK stage's x sweep writes into its output (RK2 / RK3):
-// the next stage then needs no ghost fill. Off for grids under 5 points a side.
-GhostOut stage_ghosts(const wenofd_solver* h) {
-    GhostOut gh;
-    gh.on = h->g.nx >= 5 && h->g.ny >= 5;
-    gh.bc = h->bc;
-    gh.special = h->special;
-    return gh;
 }
 
 int run_rk(wenofd_solver* h, int rkmeth, double dt) {
@@ -519,11 +561,11 @@ int enqueue_device_step(wenofd_solver* h, const BatchView& v, bool first) {
         if ((fill || !k.gh.on) && (rc = chk(launch_ghost_fill(in, h->special, h->g, h->bc, h->stream, st))))
             return rc;
         const int fp = h->opt.fp_contract;
-        if ((rc = chk(WENOFD_FP(fp, launch_flux_y(h->opt.scheme, h->tiles, in, h->Fy, h->g, h->stream, nullptr, st)))))
+        if ((rc = chk(WENOFD_FP(fp, launch_flux_y(h->opt.scheme, h->tiles, in, h->Fy, h->g, h->stream, st)))))
             return rc;
         return chk(WENOFD_FP(fp, launch_flux_x_rk(h->opt.scheme, h->tiles, in, h->Fy, un, out, nullptr, h->g, k,
                                                   h->opt.cfl, last ? h->dt_part : nullptr,
-                                                  last ? h->dt_part_nan : nullptr, h->stream, nullptr, st)));
+                                                  last ? h->dt_part_nan : nullptr, h->stream, st)));
     };
     int rc;
     if ((rc = chk(launch_batch_dt(v, h->stream)))) return rc;
@@ -717,6 +759,9 @@ int wenofd_destroy(wenofd_solver* h) {
     (void)hipFree(h->fl_raw_y);
     (void)hipFree(h->fl_x);
     (void)hipFree(h->fl_y);
+    for (void* p : {(void*)h->ent_x, (void*)h->ent_y, (void*)h->slot_x, (void*)h->slot_y, (void*)h->corr_x,
+                    (void*)h->corr_y, (void*)h->n_pairs, (void*)h->list_cnt, (void*)h->list_off})
+        (void)hipFree(p);
     if (h->dt_host) (void)hipHostFree(h->dt_host);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -1071,18 +1116,31 @@ int wenofd_switch_set(wenofd_solver* h, int mode, const float* params, float thr
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (!h->fl_x) {
-        HIP_TRY(hipMalloc(&h->fl_raw_x, h->faces_x()));
-        HIP_TRY(hipMalloc(&h->fl_raw_y, h->faces_y()));
-        HIP_TRY(hipMalloc(&h->fl_x, h->faces_x()));
-        HIP_TRY(hipMalloc(&h->fl_y, h->faces_y()));
+        const long fx = h->faces_x(), fy = h->faces_y();
+        HIP_TRY(hipMalloc(&h->fl_raw_x, fx));
+        HIP_TRY(hipMalloc(&h->fl_raw_y, fy));
+        HIP_TRY(hipMalloc(&h->fl_x, fx));
+        HIP_TRY(hipMalloc(&h->fl_y, fy));
+        // up to every face flagged
+        HIP_TRY(hipMalloc(&h->ent_x, (size_t)fx * sizeof(unsigned)));
+        HIP_TRY(hipMalloc(&h->ent_y, (size_t)fy * sizeof(unsigned)));
+        HIP_TRY(hipMalloc(&h->slot_x, (size_t)fx * sizeof(unsigned)));
+        HIP_TRY(hipMalloc(&h->slot_y, (size_t)fy * sizeof(unsigned)));
+        HIP_TRY(hipMalloc(&h->corr_x, 4 * (size_t)fx * sizeof(double)));
+        HIP_TRY(hipMalloc(&h->corr_y, 4 * (size_t)fy * sizeof(double)));
+        HIP_TRY(hipMalloc(&h->n_pairs, 2 * sizeof(int)));
+        const long nb = std::max(list_blocks(fx), list_blocks(fy));
+        HIP_TRY(hipMalloc(&h->list_cnt, 4 * nb * sizeof(int)));
+        HIP_TRY(hipMalloc(&h->list_off, 4 * nb * sizeof(int)));
     }
     if (mode == WENOFD_SWITCH_MLP) {
         if (!h->sw_params) HIP_TRY(hipMalloc(&h->sw_params, kNParams * sizeof(float)));
         HIP_TRY(hipMemcpyAsync(h->sw_params, params, kNParams * sizeof(float), hipMemcpyHostToDevice, h->stream));
     } else if (mode != WENOFD_SWITCH_OFF) {
-        const unsigned char v = mode == WENOFD_SWITCH_WENO ? 1 : 0;
+        const unsigned char v = mode == WENOFD_SWITCH_WENO ? 15 : 0;  // every field flagged, or none
         HIP_TRY(launch_fill_u8(h->fl_x, h->faces_x(), v, h->stream));
         HIP_TRY(launch_fill_u8(h->fl_y, h->faces_y(), v, h->stream));
+        HIP_TRY(build_lists(h));
     }
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->sw_mode = mode;

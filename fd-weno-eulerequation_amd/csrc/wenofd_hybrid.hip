@@ -1,0 +1,479 @@
+// wenofd_hybrid.hip — the learned switch's RK stage: one kernel per stage that
+// computes every face flux of both directions, the rhs, the RK update, the
+// CFL partials of the output and its ghost points. No counterpart in the
+// reference (the hybrid flux is this project's; the plain stage is
+// src/WENOFD.cpp:481-798 + :352-396).
+//
+// Flux at each face (hybrid_face in the oracle): the cheap component-wise
+// upwind-5 flux, plus, for each characteristic field k flagged in the face's
+// mask (bit k), R[:,k] (w_k - l_k) with w_k the scheme's WENO5 reconstruction
+// of field k and l_k its linear reconstruction from the same projected
+// stencils. Unflagged fields cost nothing beyond the cheap flux.
+//
+// Two kernels per stage. k_hyb_patch runs first, one face of the list of
+// faces with flagged fields per thread (the list is built once per
+// classification): the face's six stencil points (L2), their split, the mean
+// state's eigen parts, then per flagged field row k of L, column k of R, the
+// projections, the WENO and linear reconstructions and the correction. k_hyb_stage then does the rest, one wave per block: a block
+// owns a strip of 64 loaded columns (58 output columns, 3 halo columns each
+// side) and a segment of H output rows; lane l owns column i = c0 - 3 + l
+// and marches up the rows.
+//   y faces: each loaded row's y split is added into the partial sums of the
+//            six faces it reaches (registers; a face completes once the row
+//            two above it is in), so each point is split once per direction;
+//   x faces: the row's x split goes through wave-local LDS to the lanes of
+//            the faces that read it;
+//   a face with flagged fields adds their summed correction (its slot: its
+//   position in the list), prefetched a row ahead with the slots.
+// Compiled twice, like wenofd_kernels.hip: exact (no contraction) and fused.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "wenofd_index.hpp"
+#include "wenofd_kernels.hpp"
+#include "wenofd_math.hpp"
+#include "wenofd_switch.hpp"
+
+#ifndef WENOFD_FP_NS
+#error "build with -DWENOFD_FP_NS=exact (with -ffp-contract=off) or -DWENOFD_FP_NS=fused (with -ffp-contract=fast)"
+#endif
+
+namespace wenofd {
+namespace WENOFD_FP_NS {
+
+constexpr int kHybCols = 64;             // loaded columns per strip = lanes
+constexpr int kHybOut = kHybCols - 6;    // output columns per strip
+constexpr int kHybRows = 16;             // output rows per block
+
+__host__ __device__ inline int hyb_strips(const Geom& g) { return (g.nx + kHybOut - 1) / kHybOut; }
+__host__ __device__ inline int hyb_segments(const Geom& g) { return (g.ny + kHybRows - 1) / kHybRows; }
+
+// The cell terms with the spacings (as the plain x sweep: the build that
+// divides by the reciprocal multiplies by the host's 1/dx and 1/dy).
+__device__ __forceinline__ double hyb_cell_rhs(double fxl, double fxr, double fyb, double fyt, const Geom& g) {
+#if WENOFD_RECIP_DIV
+    return -(fxr - fxl) * g.rdx + -(fyt - fyb) * g.rdy;
+#else
+    return cell_rhs(fxl, fxr, fyb, fyt, g.dx, g.dy);
+#endif
+}
+
+__device__ __forceinline__ double hyb_local_dt(const double U[4], const Geom& g, double cfl) {
+#if WENOFD_RECIP_DIV
+    const Prim<double> q = primitive(U[0], U[1], U[2], U[3], g.gamma);
+    return cfl / ((fabs(q.u) + q.c) * g.rdx + (fabs(q.v) + q.c) * g.rdy);
+#else
+    return local_dt(U, g.gamma, cfl, g.dx, g.dy);
+#endif
+}
+
+__device__ __forceinline__ double hyb_wave_min(double v) {
+    // every lane active; MIN is exact, so the exchange order does not matter
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// ((L0 f0 + L1 f1) + L2 f2) + L3 f3, every coefficient multiplied (the
+// oracle's 4-term sum, src/WENOFD.cpp:552-593).
+__device__ __forceinline__ double dot4(const double a[4], const double f[4]) {
+    return ((a[0] * f[0] + a[1] * f[1]) + a[2] * f[2]) + a[3] * f[3];
+}
+
+// w - l of one field from its projected stencils vp[0..4] (points f-3..f+1 of
+// F+) and vm[0..4] (points f-2..f+2 of F-): the scheme's WENO5 (the shared
+// reciprocal fast path, the IEEE divisions where a lane's operands leave its
+// range) minus the linear reconstruction.
+template <int S>
+__device__ __forceinline__ double field_departure(const double vp[5], const double vm[5], double zp_lambda) {
+    bool ok = true;
+    double w = weno5_shared_rcp<S>(vp[0], vp[1], vp[2], vp[3], vp[4], zp_lambda, ok) +
+               weno5_shared_rcp<S>(vm[4], vm[3], vm[2], vm[1], vm[0], zp_lambda, ok);
+    if (__any(!ok)) {
+        const double we = weno5<S>(vp[0], vp[1], vp[2], vp[3], vp[4], zp_lambda) +
+                          weno5<S>(vm[4], vm[3], vm[2], vm[1], vm[0], zp_lambda);
+        if (!ok) w = we;
+    }
+    const double l = linear5(vp[0], vp[1], vp[2], vp[3], vp[4]) + linear5(vm[4], vm[3], vm[2], vm[1], vm[0]);
+    return w - l;
+}
+
+// Cheap-flux contribution of one point at offset o = p - f (o = -3..2) to
+// face f: c+_o F+ + c-_o F- (upwind-5 on F+ and its mirror on F-).
+template <int O>
+__device__ __forceinline__ double cheap_term(double fp, double fm) {
+    if constexpr (O == -3) return 2.0 * fp;
+    else if constexpr (O == -2) return -13.0 * fp + -3.0 * fm;
+    else if constexpr (O == -1) return 47.0 * fp + 27.0 * fm;
+    else if constexpr (O == 0) return 27.0 * fp + 47.0 * fm;
+    else if constexpr (O == 1) return -3.0 * fp + -13.0 * fm;
+    else return 2.0 * fm;
+}
+
+// Orders this wave's LDS accesses around an exchange between its lanes. The
+// stage kernel's blocks are single waves, whose LDS operations complete in
+// issue order, so only the compiler must not move accesses across it: no
+// barrier, and no wait for outstanding global loads (which a workgroup fence
+// would drain, serialising the row march's prefetches).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// Corrections of the flagged fields, one face per thread: the face's six
+// stencil points in Uin (read from L2: each list group runs in face order)
+// split along D once, the mean state's eigen parts once, then for each
+// flagged field k in ascending order R[:,k] (w_k - l_k), summed. The list
+// groups faces by their number of flagged fields, so the lanes of a wave run
+// the same number of fields. Blocks [0, nb) take the x list, [nb, 2 nb) the y
+// list (grid-stride, a fixed launch shape).
+// ---------------------------------------------------------------------------
+constexpr int kPatchThreads = 256;
+constexpr int kPatchBlocks = 512;  // per direction
+
+template <int S, int D>
+__device__ __forceinline__ void patch_face(const double* __restrict__ Uin, const Geom& g, unsigned face, unsigned m,
+                                           double* __restrict__ out) {
+    size_t p0, step;  // index of stencil point f-3 and the stride between points
+    if constexpr (D == 0) {
+        const int row = (int)(face / (unsigned)(g.nx + 1)), f = (int)(face % (unsigned)(g.nx + 1)) + 4;
+        p0 = (size_t)(row + 4) * g.pitch + (f - 3);
+        step = 1;
+    } else {
+        const int fr = (int)(face / (unsigned)g.nx) + 4, i = (int)(face % (unsigned)g.nx) + 4;
+        p0 = (size_t)(fr - 3) * g.pitch + i;
+        step = (size_t)g.pitch;
+    }
+    double Fp[6][4], Fm[6][4], ave[4];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        double P[4];
+        const size_t o = p0 + (size_t)s * step;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) P[r] = Uin[r * g.plane + o];
+        if (s == 2) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) ave[r] = P[r];
+        }
+        if (s == 3) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) ave[r] = 0.5 * (ave[r] + P[r]);
+        }
+        flux_split<D>(P, g.gamma, Fm[s], Fp[s]);
+    }
+    const EigenParts<double> e = eigen_parts<D>(ave, g.gamma);
+    double corr[4];
+    bool first = true;
+#pragma unroll 1
+    for (unsigned rest = m; rest; rest &= rest - 1) {
+        const int kf = __builtin_ctz(rest);
+        double Lk[4], Rk[4], vp[5], vm[5];
+        eigen_row_col<D>(e, kf, Lk, Rk);
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            vp[s] = dot4(Lk, Fp[s]);
+            vm[s] = dot4(Lk, Fm[s + 1]);
+        }
+        const double d = field_departure<S>(vp, vm, g.zp_lambda);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const double c = Rk[r] * d;
+            corr[r] = first ? c : corr[r] + c;
+        }
+        first = false;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[r] = corr[r];
+}
+
+template <int S>
+__global__ __launch_bounds__(kPatchThreads) void k_hyb_patch(const double* __restrict__ Uin, Geom g,
+                                                             const unsigned* __restrict__ list_x,
+                                                             const unsigned* __restrict__ list_y,
+                                                             const unsigned char* __restrict__ mx,
+                                                             const unsigned char* __restrict__ my,
+                                                             const int* __restrict__ n_dev,
+                                                             double* __restrict__ corr_x,
+                                                             double* __restrict__ corr_y,
+                                                             const SolveState* __restrict__ st) {
+    if (st && !st->active) return;
+    const int nb = gridDim.x / 2;
+    const bool xdir = (int)blockIdx.x < nb;
+    const int b = xdir ? blockIdx.x : blockIdx.x - nb;
+    const int n = n_dev[xdir ? 0 : 1];
+    for (int e = b * kPatchThreads + threadIdx.x; e < n; e += nb * kPatchThreads) {
+        if (xdir) {
+            const unsigned face = list_x[e];
+            patch_face<S, 0>(Uin, g, face, mx[face] & 15u, corr_x + (size_t)e * 4);
+        } else {
+            const unsigned face = list_y[e];
+            patch_face<S, 1>(Uin, g, face, my[face] & 15u, corr_y + (size_t)e * 4);
+        }
+    }
+}
+
+// The summed correction of a face's flagged fields (loaded when its slot is
+// not ~0).
+__device__ __forceinline__ void load_correction(unsigned slot, const double* __restrict__ corr, double c[4]) {
+    if (slot != ~0u) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) c[r] = corr[(size_t)slot * 4 + r];
+    }
+}
+
+// One RK stage of the hybrid solver on strip (bx, by):
+//   kRkNone:  rhs only (rhs_out)     kRk1: out = in + dt rhs
+//   kRkStage: out = (a Un + b in) + (c dt) rhs
+// slot_x / slot_y: per face (x faces [ny][nx+1], y faces [ny+1][nx]) the
+// slot of its flagged fields' corrections in corr_x / corr_y. With part set,
+// the block's CFL partial minimum of its updated cells.
+template <int H>
+__global__ __launch_bounds__(kHybCols, 2) void k_hyb_stage(const double* __restrict__ Uin, const double* Un,
+                                                             double* Uout, double* __restrict__ rhs_out, Geom g,
+                                                             RkCoef k, double cfl, double* __restrict__ part,
+                                                             int* __restrict__ part_nan,
+                                                             const unsigned* __restrict__ slot_x,
+                                                             const unsigned* __restrict__ slot_y,
+                                                             const double* __restrict__ corr_x,
+                                                             const double* __restrict__ corr_y,
+                                                             const SolveState* __restrict__ st) {
+    if (st) {
+        if (!st->active) return;
+        k.cdt = k.cdt * st->dt;
+    }
+    __shared__ double sxp[4][kHybCols], sxm[4][kHybCols], sfx[4][kHybCols];
+    const int lane = threadIdx.x;
+    const int c0 = 4 + blockIdx.x * kHybOut;  // first output column
+    const int i = c0 - 3 + lane;              // this lane's column
+    const bool col_in = i < g.nx + 8;
+    const bool out_col = lane >= 3 && lane < 3 + kHybOut && i < g.nx + 4;
+    const bool x_face = lane >= 3 && lane <= 3 + kHybOut && i <= g.nx + 4;  // face i (points i-1, i)
+    const int r0 = 4 + blockIdx.y * H;
+    const int r1 = r0 + H < g.ny + 4 ? r0 + H : g.ny + 4;
+    const bool gh_x = k.gh.on && ghost_x_edge(g, c0, c0 + kHybOut - 1);
+
+    auto load_u = [&](int row, double u[4]) {
+        if (col_in) {
+            const size_t o = (size_t)row * g.pitch + i;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) u[r] = Uin[r * g.plane + o];
+        } else {  // a harmless state in columns past the array (never used)
+            u[0] = 1.0;
+            u[1] = u[2] = 0.0;
+            u[3] = 2.5;
+        }
+    };
+    auto load_un = [&](int row, double u[4]) {
+        if (out_col && k.mode == kRkStage && row < r1) {
+            const size_t o = (size_t)row * g.pitch + i;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) u[r] = Un[r * g.plane + o];
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) u[r] = 0.0;
+        }
+    };
+
+    auto slot_of_y = [&](int face_row) {  // face_row in [r0, r1]
+        return out_col ? slot_y[(size_t)(face_row - 4) * g.nx + (i - 4)] : ~0u;
+    };
+    auto slot_of_x = [&](int row) {  // row in [r0, r1)
+        return x_face ? slot_x[(size_t)(row - 4) * (g.nx + 1) + (i - 4)] : ~0u;
+    };
+
+    double acc[5][4];   // acc[m]: partial cheap flux of face q-2+m (before row q)
+    double fyb[4];      // flux of the bottom face of the next cell row
+    double u1[4], u2[4];  // prefetched U of rows q+1, q+2
+    double ucn[4], unn[4];  // prefetched U and Un of the next cell row
+    // prefetched slots and corrections: y faces fc+1 (slot + values) and fc+2
+    // (slot), x faces of cell rows c+1 (slot + values) and c+2 (slot)
+    unsigned sy1 = slot_of_y(r0), sy2 = slot_of_y(r0 + 1 <= r1 ? r0 + 1 : r1);
+    unsigned sx1 = slot_of_x(r0), sx2 = r0 + 1 < r1 ? slot_of_x(r0 + 1) : ~0u;
+    double cy1[4], cx1[4];
+    load_correction(sy1, corr_y, cy1);
+    load_correction(sx1, corr_x, cx1);
+    double my_min = 1.0;
+    int my_nan = 0;
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[m][r] = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) fyb[r] = 0.0;
+    load_u(r0 - 3, u1);
+    load_u(r0 - 2, u2);
+    load_u(r0, ucn);
+    load_un(r0, unn);
+
+    for (int q = r0 - 3; q <= r1 + 2; ++q) {
+        // ---- row q: y split, partial sums of faces q-2 .. q+3 ----
+        double u[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            u[r] = u1[r];
+            u1[r] = u2[r];
+        }
+        if (q + 2 <= r1 + 2) load_u(q + 2, u2);
+        double fm[4], fp[4], fyt[4];
+        flux_split<1>(u, g.gamma, fm, fp);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            fyt[r] = (acc[0][r] + cheap_term<2>(fp[r], fm[r])) * (1.0 / 60.0);  // face q-2 complete
+            acc[0][r] = acc[1][r] + cheap_term<1>(fp[r], fm[r]);
+            acc[1][r] = acc[2][r] + cheap_term<0>(fp[r], fm[r]);
+            acc[2][r] = acc[3][r] + cheap_term<-1>(fp[r], fm[r]);
+            acc[3][r] = acc[4][r] + cheap_term<-2>(fp[r], fm[r]);
+            acc[4][r] = cheap_term<-3>(fp[r], fm[r]);
+        }
+        const int fc = q - 2;  // the face just completed (wave-uniform)
+        if (fc < r0) continue;
+        if (sy1 != ~0u) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) fyt[r] = fyt[r] + cy1[r];
+        }
+        sy1 = sy2;
+        load_correction(sy1, corr_y, cy1);
+        if (fc + 2 <= r1) sy2 = slot_of_y(fc + 2);
+        if (fc == r0) {  // the bottom face of the first cell row
+#pragma unroll
+            for (int r = 0; r < 4; ++r) fyb[r] = fyt[r];
+            continue;
+        }
+
+        // ---- cell row c = fc - 1 (= q - 3): x faces, then the update ----
+        const int c = fc - 1;
+        double uc[4], un[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            uc[r] = ucn[r];
+            un[r] = unn[r];
+        }
+        if (c + 1 < r1) {
+            load_u(c + 1, ucn);
+            load_un(c + 1, unn);
+        }
+        double fxm[4], fxp[4];
+        flux_split<0>(uc, g.gamma, fxm, fxp);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            sxp[r][lane] = fxp[r];
+            sxm[r][lane] = fxm[r];
+        }
+        wave_sync();
+        if (x_face) {  // face i: F+ of lanes lane-3 .. lane+1, F- of lane-2 .. lane+2
+            double fx[4];
+#pragma unroll 1
+            for (int r = 0; r < 4; ++r) {
+                double a = cheap_term<-3>(sxp[r][lane - 3], 0.0);
+                a = a + cheap_term<-2>(sxp[r][lane - 2], sxm[r][lane - 2]);
+                a = a + cheap_term<-1>(sxp[r][lane - 1], sxm[r][lane - 1]);
+                a = a + cheap_term<0>(sxp[r][lane], sxm[r][lane]);
+                a = a + cheap_term<1>(sxp[r][lane + 1], sxm[r][lane + 1]);
+                a = a + cheap_term<2>(0.0, sxm[r][lane + 2]);
+                fx[r] = a * (1.0 / 60.0);
+            }
+            if (sx1 != ~0u) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) fx[r] = fx[r] + cx1[r];
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sfx[r][lane] = fx[r];
+        }
+        sx1 = sx2;
+        load_correction(sx1, corr_x, cx1);
+        sx2 = c + 2 < r1 ? slot_of_x(c + 2) : ~0u;
+        wave_sync();
+        if (out_col) {
+            double rhs[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) rhs[r] = hyb_cell_rhs(sfx[r][lane], sfx[r][lane + 1], fyb[r], fyt[r], g);
+            if (k.gravity) {  // src/WENOFD.cpp:812-813
+                rhs[3] = rhs[3] + uc[2];
+                rhs[2] = rhs[2] + uc[0];
+            }
+            const size_t o = (size_t)c * g.pitch + i;
+            if (rhs_out) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) rhs_out[r * g.plane + o] = rhs[r];
+            }
+            if (k.mode != kRkNone) {
+                double nu[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    nu[r] = (k.mode == kRk1) ? uc[r] + k.cdt * rhs[r]
+                                             : rk_combine(k.a, un[r], k.b, uc[r], k.cdt, rhs[r]);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Uout[r * g.plane + o] = nu[r];
+                const bool gh_y = k.gh.on && ghost_y_edge(g, c);
+                if (gh_x || gh_y) {
+                    ghost_targets(
+                        g, k.gh.bc, i, c, gh_x, gh_y,
+                        [&](int di, int dj, int flip) {
+                            const size_t d = (size_t)dj * g.pitch + di;
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) Uout[r * g.plane + d] = (r == flip) ? -nu[r] : nu[r];
+                        },
+                        [&](int di, int dj, int spec) {
+                            const size_t d = (size_t)dj * g.pitch + di;
+                            const int ns = 2 * (g.nx + g.ny);
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) Uout[r * g.plane + d] = k.gh.special[r * ns + spec];
+                        });
+                }
+                if (part) {
+                    const double tl = hyb_local_dt(nu, g, cfl);
+                    if (tl != tl) my_nan = 1;
+                    my_min = (tl < my_min) ? tl : my_min;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) fyb[r] = fyt[r];
+    }
+    if (part) {
+        my_min = hyb_wave_min(my_min);
+        my_nan = __any(my_nan) ? 1 : 0;
+        if (lane == 0) {
+            const int b = blockIdx.y * gridDim.x + blockIdx.x;
+            part[b] = my_min;
+            part_nan[b] = my_nan;
+        }
+    }
+}
+
+long hyb_stage_blocks(const Geom& g) { return (long)hyb_strips(g) * hyb_segments(g); }
+
+hipError_t launch_hyb_patch(int scheme, const double* Uin, const Geom& g, const unsigned* list_x,
+                            const unsigned* list_y, const unsigned char* mx, const unsigned char* my,
+                            const int* n_dev, double* corr_x, double* corr_y, hipStream_t s, const SolveState* st) {
+    auto go = [&](auto S) {
+        hipLaunchKernelGGL((k_hyb_patch<decltype(S)::value>), dim3(2 * kPatchBlocks), dim3(kPatchThreads), 0, s,
+                           Uin, g, list_x, list_y, mx, my, n_dev, corr_x, corr_y, st);
+    };
+    switch (scheme) {
+        case kJS: go(std::integral_constant<int, kJS>()); break;
+        case kZ: go(std::integral_constant<int, kZ>()); break;
+        case kZP: go(std::integral_constant<int, kZP>()); break;
+        default: go(std::integral_constant<int, kZPI>()); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_hyb_stage(const double* Uin, const double* Un, double* Uout, double* rhs_out, const Geom& g,
+                            const RkCoef& k, double cfl, double* dt_part, int* dt_nan, const unsigned* slot_x,
+                            const unsigned* slot_y, const double* corr_x, const double* corr_y, hipStream_t s,
+                            const SolveState* st) {
+    const dim3 grid(hyb_strips(g), hyb_segments(g));
+    hipLaunchKernelGGL((k_hyb_stage<kHybRows>), grid, dim3(kHybCols), 0, s, Uin, Un, Uout, rhs_out, g, k, cfl,
+                       dt_part, dt_nan, slot_x, slot_y, corr_x, corr_y, st);
+    return hipGetLastError();
+}
+
+}  // namespace WENOFD_FP_NS
+}  // namespace wenofd

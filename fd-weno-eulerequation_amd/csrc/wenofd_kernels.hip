@@ -119,15 +119,9 @@ __global__ __launch_bounds__(kGhostThreads) void k_ghost_fill(double* __restrict
 // (kYR face rows per step, the kRing-row split ring and the kRawRing-row point
 // ring: wenofd_index.hpp, where a host check replays their schedule.)
 
-// HYB: faces whose flag (flags[(f-4) nx + (i-4)]) is zero take the
-// component-wise upwind-5 flux; the flagged ones take the characteristic WENO
-// flux in their own lanes, so a wave runs it only if one of its 64 faces is
-// flagged. (Gathering the flagged faces of a step onto the fewest waves makes
-// the WENO flux's column a per-lane value: 182 VGPRs, 2 waves per SIMD, and a
-// slower sweep at every flag rate measured.)
-template <int S, int YF, bool HYB = false>
+template <int S, int YF>
 __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double* __restrict__ Fy, const Geom& g,
-                                            int bx, int by, const unsigned char* __restrict__ flags = nullptr) {
+                                            int bx, int by) {
     static_assert(YF % kYR == 0, "YF must be a multiple of the step height");
     __shared__ double sP[4][kRing][kYC];
     __shared__ double sM[4][kRing][kYC];
@@ -185,24 +179,7 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
             for (int r = 0; r < 4; ++r) Fy[r * g.fy_plane + o] = Fh[r];
         };
         const int fr = f + ty;
-        if constexpr (HYB) {
-            bool fl = false;
-            if (col_ok && fr < F1) {
-                fl = flags[(size_t)(fr - 4) * g.nx + (i - 4)] != 0;
-                if (!fl) {
-                    const int s0 = fr - 3 - base;
-                    double Fh[4];
-                    face_flux_linear<double>([&](int s, int r) { return sP[r][(s0 + s) % kRing][tx]; },
-                                             [&](int s, int r) { return sM[r][(s0 + s) % kRing][tx]; }, Fh);
-                    const size_t o = (size_t)(fr - 4) * g.fy_pitch + (i - 4);
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) Fy[r * g.fy_plane + o] = Fh[r];
-                }
-            }
-            if (fl) full_face(tx, fr);
-        } else {
-            if (col_ok && fr < F1) full_face(tx, fr);
-        }
+        if (col_ok && fr < F1) full_face(tx, fr);
         __syncthreads();
     }
 }
@@ -213,16 +190,15 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
 // divides by the reciprocal caps every scheme: JS and Z would otherwise take
 // 246 VGPRs, and its ZPI sweep runs 4 % faster capped (a 9-dword spill) than
 // at 2 waves.
-template <int S, bool HYB>
-constexpr int y_min_waves() { return !HYB && (S == kZP || kCapRecip) ? 3 : 1; }
+template <int S>
+constexpr int y_min_waves() { return S == kZP || kCapRecip ? 3 : 1; }
 
-template <int S, int YF, bool HYB>
-__global__ __launch_bounds__(kYC* kYR, (y_min_waves<S, HYB>())) void k_flux_y(const double* __restrict__ U,
-                                                                            double* __restrict__ Fy, Geom g,
-                                                     const unsigned char* __restrict__ flags,
-                                                     const SolveState* __restrict__ st) {
+template <int S, int YF>
+__global__ __launch_bounds__(kYC* kYR, (y_min_waves<S>())) void k_flux_y(const double* __restrict__ U,
+                                                                       double* __restrict__ Fy, Geom g,
+                                                                       const SolveState* __restrict__ st) {
     if (st && !st->active) return;
-    flux_y_tile<S, YF, HYB>(U, Fy, g, blockIdx.x, blockIdx.y, flags);
+    flux_y_tile<S, YF>(U, Fy, g, blockIdx.x, blockIdx.y);
 }
 
 // Minimum over the wave's 64 lanes of values that are never NaN, returned to
@@ -291,12 +267,11 @@ __device__ __forceinline__ double local_dt_g(const double U[4], const Geom& g, d
 // Cells per x tile of XT threads (x_cells, wenofd_index.hpp): XT - 5 faces
 // read exactly XT points, so every thread splits one point.
 
-template <int S, int XT, bool HYB = false>
+template <int S, int XT>
 __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, const double* __restrict__ Fy,
                                                const double* Un, double* Uout, double* __restrict__ rhs_out,
                                                const Geom& g, const RkCoef& k, double cfl, double* part,
-                                               int* part_nan, int bx, int row,
-                                               const unsigned char* __restrict__ flags = nullptr) {
+                                               int* part_nan, int bx, int row) {
     constexpr int XC = x_cells<XT>();  // cells per tile
     constexpr int XP = XC + 6;         // points per tile (= XT)
     constexpr int XS = XT + 8;  // LDS row stride (slots)
@@ -346,22 +321,7 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
 #pragma unroll
         for (int r = 0; r < 4; ++r) sF[r][tt] = Fh[r];
     };
-    if constexpr (HYB) {
-        bool fl = false;
-        if (t <= XC && f <= g.nx + 4) {
-            fl = flags[(size_t)row * (g.nx + 1) + (f - 4)] != 0;
-            if (!fl) {
-                double Fh[4];
-                face_flux_linear<double>([&](int s, int r) { return sP[r][t + s]; },
-                                         [&](int s, int r) { return sM[r][t + s]; }, Fh);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) sF[r][t] = Fh[r];
-            }
-        }
-        if (fl) full_face(t);
-    } else {
-        if (t <= XC && f <= g.nx + 4) full_face(t);
-    }
+    if (t <= XC && f <= g.nx + 4) full_face(t);
     const int i = f0 + t;  // this thread's cell
     const bool cell_ok = t < XC && i < g.nx + 4;
     // ghost points of Uout to write: only the waves whose cells reach within 5
@@ -433,29 +393,27 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
 }
 
 // The contracted WENO-ZPI x sweep would take 170 VGPRs and run at 2 waves per
-// SIMD; capped at 3 it fits without spilling (-10 % sweep time), as does the
-// switched JS / Z sweep with division by the reciprocal (246 otherwise). The
-// other sweeps fit by themselves (their code is left to the default
-// heuristics).
-template <int S, int XT, bool HYB>
-constexpr int x_min_waves() { return (S == kZPI && !HYB) || (kCapRecip && (!HYB || S <= kZ)) ? 3 : 1; }
+// SIMD; capped at 3 it fits without spilling (-10 % sweep time), as does every
+// sweep of the build that divides by the reciprocal. The other sweeps fit by
+// themselves (their code is left to the default heuristics).
+template <int S, int XT>
+constexpr int x_min_waves() { return S == kZPI || kCapRecip ? 3 : 1; }
 
 // With st, k.cdt holds the stage coefficient c and the step is c * st->dt.
-template <int S, int XT, bool HYB>
-__global__ __launch_bounds__(XT, (x_min_waves<S, XT, HYB>())) void k_flux_x_rk(const double* __restrict__ Uin,
-                                                                               const double* __restrict__ Fy,
+template <int S, int XT>
+__global__ __launch_bounds__(XT, (x_min_waves<S, XT>())) void k_flux_x_rk(const double* __restrict__ Uin,
+                                                                          const double* __restrict__ Fy,
                                                   const double* Un, double* Uout, double* __restrict__ rhs_out,
                                                   Geom g, RkCoef k, double cfl, double* __restrict__ dt_part,
-                                                  int* __restrict__ dt_nan, const unsigned char* __restrict__ flags,
-                                                  const SolveState* __restrict__ st) {
+                                                  int* __restrict__ dt_nan, const SolveState* __restrict__ st) {
     if (st) {
         if (!st->active) return;
         k.cdt = k.cdt * st->dt;
     }
     const int b = blockIdx.y * gridDim.x + blockIdx.x;
     const int lb = xcd_block(b, gridDim.x * gridDim.y);  // consecutive rows of tiles on one XCD
-    flux_x_rk_tile<S, XT, HYB>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part ? dt_part + b : nullptr,
-                               dt_part ? dt_nan + b : nullptr, lb % gridDim.x, lb / gridDim.x, flags);
+    flux_x_rk_tile<S, XT>(Uin, Fy, Un, Uout, rhs_out, g, k, cfl, dt_part ? dt_part + b : nullptr,
+                          dt_part ? dt_nan + b : nullptr, lb % gridDim.x, lb / gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -575,7 +533,7 @@ __global__ __launch_bounds__(kGhostThreads) void k_batch_ghost(BatchView b, int 
 }
 
 template <int S, int YF>
-__global__ __launch_bounds__(kYC* kYR, (y_min_waves<S, false>())) void k_batch_flux_y(BatchView b, int buf,
+__global__ __launch_bounds__(kYC* kYR, (y_min_waves<S>())) void k_batch_flux_y(BatchView b, int buf,
                                                                               const TileRef* __restrict__ tiles) {
     TileRef tr;
     SolveDesc d;
@@ -585,7 +543,7 @@ __global__ __launch_bounds__(kYC* kYR, (y_min_waves<S, false>())) void k_batch_f
 }
 
 template <int S, int XT>
-__global__ __launch_bounds__(XT, (x_min_waves<S, XT, false>())) void k_batch_flux_x_rk(BatchView b, int in, int out,
+__global__ __launch_bounds__(XT, (x_min_waves<S, XT>())) void k_batch_flux_x_rk(BatchView b, int in, int out,
                                                                                         int mode, double a,
                                                         double bcoef, double c_stage, int last, int ghosts,
                                                         const TileRef* __restrict__ tiles) {
@@ -827,31 +785,23 @@ hipError_t launch_add_source(const double* U, double* rhs, const Geom& g, hipStr
 }
 
 hipError_t launch_flux_y(int scheme, const Tiles& t, const double* U, double* Fy, const Geom& g, hipStream_t s,
-                         const unsigned char* flags, const SolveState* st) {
+                         const SolveState* st) {
     return with_scheme(scheme, [&](auto S) {
         with_y_tile(t.y, [&](auto YF) {
             dim3 grid(y_tiles_x<YF()>(g), y_tiles_y<YF()>(g));
-            if (flags)
-                hipLaunchKernelGGL((k_flux_y<S(), YF(), true>), grid, dim3(kYC * kYR), 0, s, U, Fy, g, flags, st);
-            else
-                hipLaunchKernelGGL((k_flux_y<S(), YF(), false>), grid, dim3(kYC * kYR), 0, s, U, Fy, g, flags, st);
+            hipLaunchKernelGGL((k_flux_y<S(), YF()>), grid, dim3(kYC * kYR), 0, s, U, Fy, g, st);
         });
     });
 }
 
 hipError_t launch_flux_x_rk(int scheme, const Tiles& t, const double* Uin, const double* Fy, const double* Un,
                             double* Uout, double* rhs_out, const Geom& g, const RkCoef& k, double cfl,
-                            double* dt_part, int* dt_nan, hipStream_t s, const unsigned char* flags,
-                            const SolveState* st) {
+                            double* dt_part, int* dt_nan, hipStream_t s, const SolveState* st) {
     return with_scheme(scheme, [&](auto S) {
         with_x_tile(t.x, [&](auto XT) {
             dim3 grid(x_tiles_x<XT()>(g), g.ny);
-            if (flags)
-                hipLaunchKernelGGL((k_flux_x_rk<S(), XT(), true>), grid, dim3(XT()), 0, s, Uin, Fy, Un, Uout,
-                                   rhs_out, g, k, cfl, dt_part, dt_nan, flags, st);
-            else
-                hipLaunchKernelGGL((k_flux_x_rk<S(), XT(), false>), grid, dim3(XT()), 0, s, Uin, Fy, Un, Uout,
-                                   rhs_out, g, k, cfl, dt_part, dt_nan, flags, st);
+            hipLaunchKernelGGL((k_flux_x_rk<S(), XT()>), grid, dim3(XT()), 0, s, Uin, Fy, Un, Uout, rhs_out, g, k,
+                               cfl, dt_part, dt_nan, st);
         });
     });
 }

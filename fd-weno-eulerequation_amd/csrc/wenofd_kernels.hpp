@@ -123,6 +123,7 @@ using exact::launch_ghost_fill;
 using exact::launch_copy_interior;
 using exact::launch_add_source;
 using exact::flux_x_rk_blocks;
+using exact::hyb_stage_blocks;
 using exact::flux_y_blocks;
 using exact::ghost_blocks;
 using exact::launch_dt_final;
@@ -142,16 +143,24 @@ using exact::launch_batch_copy;
 #define WENOFD_FP(mode, call) ((mode) == ::wenofd::kFpFused ? ::wenofd::fused::call : ::wenofd::exact::call)
 
 // ---- learned switch (wenofd_switch.hip) ----------------------------------------
-// Flags (and optionally logits [faces][4]) of every face of direction dir from
-// the ghost-filled state U; params = MLP parameters (float, device).
+// Field masks (bit k: field k's logit above thr; optionally the logits
+// [faces][4]) of every face of direction dir from the ghost-filled state U;
+// params = MLP parameters (float, device).
 hipError_t launch_classify(int dir, const double* U, const Geom& g, const float* params, float thr,
                            unsigned char* flags, float* logits, hipStream_t s);
 // Features [faces][4][10] and labels [faces][4] of every face of direction dir.
 hipError_t launch_harvest(int dir, const double* U, const Geom& g, float* X, unsigned char* Y, hipStream_t s);
-// OR over the (2 rad + 1)^2 neighbourhood of each face in the rows x cols face grid.
-hipError_t launch_dilate(const unsigned char* in, unsigned char* out, int rows, int cols, int rad, bool group,
-                         hipStream_t s);
+// Bitwise OR of the field masks over the (2 rad + 1)^2 neighbourhood of each
+// face in the rows x cols face grid.
+hipError_t launch_dilate(const unsigned char* in, unsigned char* out, int rows, int cols, int rad, hipStream_t s);
 hipError_t launch_fill_u8(unsigned char* p, long n, unsigned char v, hipStream_t s);
+// The faces of one direction's masks (F faces) that flag any field: list[]
+// grouped by the number of flagged fields (1 .. 4), each group in face order;
+// slot[face] = its position in list, or ~0; *n = the number of faces (device).
+// cnt / off: 4 list_blocks(F) ints of scratch each.
+long list_blocks(long F);
+hipError_t launch_build_list(const unsigned char* masks, long F, int* cnt, int* off, unsigned* list, unsigned* slot,
+                             int* n, hipStream_t s);
 hipError_t launch_mlp_infer(const float* P, const float* X, long n, float* z, hipStream_t s);
 hipError_t launch_keep_count(const unsigned char* Y, long n, unsigned long long seed, double p_pos, double p_neg,
                              int* counts, hipStream_t s);

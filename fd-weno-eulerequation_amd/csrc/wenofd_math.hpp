@@ -350,53 +350,109 @@ WENOFD_HD void flux_split(const T U[4], double gamma, T Fm[4], T Fp[4]) {
     }
 }
 
+// The quantities both eigenmatrices of dF/dU along D are built from, at state
+// U, with the reference's expression forms (utils/equations.hpp:174-299):
+// B1 = (g-1)/c^2, B2 = B1 (u^2+v^2)/2, H = (E+p)/rho, 1/c and u_n/c.
+template <class T>
+struct EigenParts {
+    T u, v, c, H, B1, B2, B1u, B1v, ic, nc, uu;  // nc = u/c (x) or v/c (y)
+};
+
+template <int D, class T>
+WENOFD_HD EigenParts<T> eigen_parts(const T U[4], double gamma) {
+    const Prim<T> q = primitive(U[0], U[1], U[2], U[3], gamma);
+    EigenParts<T> e;
+    e.u = q.u;
+    e.v = q.v;
+    e.c = q.c;
+#if WENOFD_RCP_STATE
+    e.B1 = (gamma - 1.0) * rcp_refined(q.c * q.c);
+    e.ic = rcp_refined(q.c);
+    e.H = (q.E + q.p) * q.irho;
+    e.nc = (D == 0 ? q.u : q.v) * e.ic;
+#else
+    e.B1 = (gamma - 1.0) / (q.c * q.c);
+    e.ic = 1.0 / q.c;
+    e.H = (q.E + q.p) / q.rho;
+    e.nc = (D == 0 ? q.u : q.v) / q.c;
+#endif
+    e.uu = q.u * q.u + q.v * q.v;
+    e.B2 = (e.B1 * e.uu) / 2.0;
+    e.B1u = e.B1 * q.u;
+    e.B1v = e.B1 * q.v;
+    return e;
+}
+
 // Left (rows = left eigenvectors) and right eigenmatrices of dF/dU along D at
 // state U, with the reference's normalisation (utils/equations.hpp:174-299).
 // Row-major 4x4: L[i*4+j].
 template <int D, class T>
 WENOFD_HD void eigen_matrices(const T U[4], double gamma, T L[16], T R[16]) {
-    const Prim<T> q = primitive(U[0], U[1], U[2], U[3], gamma);
-#if WENOFD_RCP_STATE
-    const T B1 = (gamma - 1.0) * rcp_refined(q.c * q.c);
-    const T ic = rcp_refined(q.c);
-    const T H = (q.E + q.p) * q.irho;
-#else
-    const T B1 = (gamma - 1.0) / (q.c * q.c);
-    const T ic = 1.0 / q.c;
-    const T H = (q.E + q.p) / q.rho;
-#endif
-    const T uu = q.u * q.u + q.v * q.v;
-    const T B2 = (B1 * uu) / 2.0;
-    const T B1u = B1 * q.u, B1v = B1 * q.v;
+    const EigenParts<T> e = eigen_parts<D>(U, gamma);
+    const T B1 = e.B1, B2 = e.B2, B1u = e.B1u, B1v = e.B1v, ic = e.ic, H = e.H, uu = e.uu;
     if constexpr (D == 0) {
-#if WENOFD_RCP_STATE
-        const T uc = q.u * ic;
-#else
-        const T uc = q.u / q.c;
-#endif
+        const T uc = e.nc;
         L[0] = (B2 + uc) / 2.0;  L[1] = -(B1u + ic) / 2.0;  L[2] = -B1v / 2.0;  L[3] = B1 / 2.0;
-        L[4] = q.v;              L[5] = 0.0;                L[6] = -1.0;        L[7] = 0.0;
+        L[4] = e.v;              L[5] = 0.0;                L[6] = -1.0;        L[7] = 0.0;
         L[8] = 1.0 - B2;         L[9] = B1u;                L[10] = B1v;        L[11] = -B1;
         L[12] = (B2 - uc) / 2.0; L[13] = -(B1u - ic) / 2.0; L[14] = -B1v / 2.0; L[15] = B1 / 2.0;
         R[0] = 1.0;              R[1] = 0.0;   R[2] = 1.0;        R[3] = 1.0;
-        R[4] = q.u - q.c;        R[5] = 0.0;   R[6] = q.u;        R[7] = q.u + q.c;
-        R[8] = q.v;              R[9] = -1.0;  R[10] = q.v;       R[11] = q.v;
-        R[12] = H - q.c * q.u;   R[13] = -q.v; R[14] = uu / 2.0;  R[15] = H + q.c * q.u;
+        R[4] = e.u - e.c;        R[5] = 0.0;   R[6] = e.u;        R[7] = e.u + e.c;
+        R[8] = e.v;              R[9] = -1.0;  R[10] = e.v;       R[11] = e.v;
+        R[12] = H - e.c * e.u;   R[13] = -e.v; R[14] = uu / 2.0;  R[15] = H + e.c * e.u;
     } else {
-#if WENOFD_RCP_STATE
-        const T vc = q.v * ic;
-#else
-        const T vc = q.v / q.c;
-#endif
+        const T vc = e.nc;
         L[0] = (B2 + vc) / 2.0;  L[1] = -B1u / 2.0; L[2] = -(B1v + ic) / 2.0;  L[3] = B1 / 2.0;
-        L[4] = -q.u;             L[5] = 1.0;        L[6] = 0.0;                L[7] = 0.0;
+        L[4] = -e.u;             L[5] = 1.0;        L[6] = 0.0;                L[7] = 0.0;
         L[8] = 1.0 - B2;         L[9] = B1u;        L[10] = B1v;               L[11] = -B1;
         L[12] = (B2 - vc) / 2.0; L[13] = -B1u / 2.0; L[14] = -(B1v - ic) / 2.0; L[15] = B1 / 2.0;
         R[0] = 1.0;              R[1] = 0.0;   R[2] = 1.0;        R[3] = 1.0;
-        R[4] = q.u;              R[5] = 1.0;   R[6] = q.u;        R[7] = q.u;
-        R[8] = q.v - q.c;        R[9] = 0.0;   R[10] = q.v;       R[11] = q.v + q.c;
-        R[12] = H - q.c * q.v;   R[13] = q.u;  R[14] = uu / 2.0;  R[15] = H + q.c * q.v;
+        R[4] = e.u;              R[5] = 1.0;   R[6] = e.u;        R[7] = e.u;
+        R[8] = e.v - e.c;        R[9] = 0.0;   R[10] = e.v;       R[11] = e.v + e.c;
+        R[12] = H - e.c * e.v;   R[13] = e.u;  R[14] = uu / 2.0;  R[15] = H + e.c * e.v;
     }
+}
+
+// Row k of L and column k of R of eigen_matrices<D> from its parts (the same
+// entries, the same expressions; k in 0..3 at run time).
+template <int D, class T>
+WENOFD_HD void eigen_row_col(const EigenParts<T>& e, int k, T Lk[4], T Rk[4]) {
+    const T B1 = e.B1, B2 = e.B2, B1u = e.B1u, B1v = e.B1v, ic = e.ic, H = e.H;
+    if (k == 1) {
+        if constexpr (D == 0) {
+            Lk[0] = e.v; Lk[1] = 0.0; Lk[2] = -1.0; Lk[3] = 0.0;
+            Rk[0] = 0.0; Rk[1] = 0.0; Rk[2] = -1.0; Rk[3] = -e.v;
+        } else {
+            Lk[0] = -e.u; Lk[1] = 1.0; Lk[2] = 0.0; Lk[3] = 0.0;
+            Rk[0] = 0.0;  Rk[1] = 1.0; Rk[2] = 0.0; Rk[3] = e.u;
+        }
+    } else if (k == 2) {
+        Lk[0] = 1.0 - B2; Lk[1] = B1u; Lk[2] = B1v; Lk[3] = -B1;
+        Rk[0] = 1.0;      Rk[1] = e.u; Rk[2] = e.v; Rk[3] = e.uu / 2.0;
+    } else {  // the acoustic fields k = 0 (s = -1) and k = 3 (s = +1)
+        const bool minus = k == 0;
+        const T cn = e.c * (D == 0 ? e.u : e.v);
+        Lk[0] = minus ? (B2 + e.nc) / 2.0 : (B2 - e.nc) / 2.0;
+        if constexpr (D == 0) {
+            Lk[1] = minus ? -(B1u + ic) / 2.0 : -(B1u - ic) / 2.0;
+            Lk[2] = -B1v / 2.0;
+            Rk[1] = minus ? e.u - e.c : e.u + e.c;
+            Rk[2] = e.v;
+        } else {
+            Lk[1] = -B1u / 2.0;
+            Lk[2] = minus ? -(B1v + ic) / 2.0 : -(B1v - ic) / 2.0;
+            Rk[1] = e.u;
+            Rk[2] = minus ? e.v - e.c : e.v + e.c;
+        }
+        Lk[3] = B1 / 2.0;
+        Rk[0] = 1.0;
+        Rk[3] = minus ? H - cn : H + cn;
+    }
+}
+
+template <int D, class T>
+WENOFD_HD void eigen_row_col(const T U[4], double gamma, int k, T Lk[4], T Rk[4]) {
+    eigen_row_col<D>(eigen_parts<D>(U, gamma), k, Lk, Rk);
 }
 
 // Projection of one split-flux vector onto the characteristic fields:

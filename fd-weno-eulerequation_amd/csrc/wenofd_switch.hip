@@ -243,41 +243,141 @@ __global__ __launch_bounds__(kSwT) void k_switch_faces(const double* __restrict_
         mlp_wave(R, sX[w], sZ[w]);
         __syncthreads();
         if (face_ok) {
-            bool fl = false;
+            unsigned char mask = 0;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const float z = sZ[w][k][lane];
-                fl = fl || z > o.thr;
+                if (z > o.thr) mask |= (unsigned char)(1u << k);
                 if (o.logits) o.logits[q * 4 + k] = z;
             }
-            o.flags[q] = fl ? 1 : 0;
+            o.flags[q] = mask;
         }
     }
 }
 
-// out[r][c] = OR of in over the (2 rad + 1)^2 neighbourhood (rows x cols grid
-// of faces). One wave per aligned group of 64 columns of a row; with `group`,
-// every face of a group takes the OR of the group's dilated flags.
+// out[r][c] = bitwise OR of the field masks of in over the (2 rad + 1)^2
+// neighbourhood (rows x cols grid of faces of one direction): each field's
+// flags dilated on their own.
 __global__ __launch_bounds__(256) void k_dilate(const unsigned char* __restrict__ in, unsigned char* __restrict__ out,
-                                                int rows, int cols, int rad, int group) {
-    const int groups = (cols + 63) / 64;
-    const long wv = (long)blockIdx.x * 4 + threadIdx.x / 64;  // wave-uniform
-    if (wv >= (long)rows * groups) return;
-    const int r = (int)(wv / groups), c = (int)(wv % groups) * 64 + (int)(threadIdx.x % 64);
-    const bool ok = c < cols;
-    bool v = false;
-    if (ok) {
-        for (int dr = -rad; dr <= rad; ++dr) {
-            const int rr = r + dr;
-            if (rr < 0 || rr >= rows) continue;
-            for (int dc = -rad; dc <= rad; ++dc) {
-                const int cc = c + dc;
-                if (cc >= 0 && cc < cols) v = v || in[(size_t)rr * cols + cc] != 0;
-            }
+                                                int rows, int cols, int rad) {
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long)rows * cols) return;
+    const int r = (int)(t / cols), c = (int)(t % cols);
+    unsigned v = 0;
+    for (int dr = -rad; dr <= rad; ++dr) {
+        const int rr = r + dr;
+        if (rr < 0 || rr >= rows) continue;
+        for (int dc = -rad; dc <= rad; ++dc) {
+            const int cc = c + dc;
+            if (cc >= 0 && cc < cols) v |= in[(size_t)rr * cols + cc];
         }
     }
-    if (group) v = __any(v);
-    if (ok) out[(size_t)r * cols + c] = v ? 1 : 0;
+    out[t] = (unsigned char)(v & 15u);
+}
+
+// ---------------------------------------------------------------------------
+// The list of one direction's faces with flagged fields, grouped by their
+// number of flagged fields (1 .. 4; within a group in face order), so that a
+// wave of the correction kernel runs one field count: k_list_count counts
+// each group in each run of kListFaces faces, k_list_scan turns the counts
+// into offsets (one block), k_list_write writes the faces and each face's
+// slot (its list position, ~0 for an unflagged face). Deterministic: every
+// position depends on the masks alone.
+// ---------------------------------------------------------------------------
+constexpr int kListFaces = 1024;  // faces per counting block (4 per thread)
+
+// Counts of the four groups packed 16 bits each (a block holds <= 1024 faces).
+__device__ __forceinline__ unsigned long long group_counts(unsigned m) {
+    return m ? 1ull << (16 * (__popc(m) - 1)) : 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_list_count(const unsigned char* __restrict__ m, long F,
+                                                    int* __restrict__ cnt) {
+    __shared__ unsigned long long s4[4];
+    const long f0 = (long)blockIdx.x * kListFaces + 4 * threadIdx.x;
+    unsigned long long c = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (f0 + q < F) c += group_counts(m[f0 + q] & 15u);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if (threadIdx.x % 64 == 0) s4[threadIdx.x / 64] = c;
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const unsigned long long t = s4[0] + s4[1] + s4[2] + s4[3];
+        cnt[4 * blockIdx.x + threadIdx.x] = (int)((t >> (16 * threadIdx.x)) & 0xFFFFu);
+    }
+}
+
+// off[4 b + g] = position of block b's first face of group g: the faces of
+// groups below g, then those of group g in the blocks before b. *n = the
+// total. One block of 1024 threads; thread t takes a contiguous run of blocks.
+__global__ __launch_bounds__(1024) void k_list_scan(const int* __restrict__ cnt, int nb, int* __restrict__ off,
+                                                    int* __restrict__ n) {
+    __shared__ int s[1024];
+    __shared__ int gbase[5];
+    const int t = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = t * per, b1 = b0 + per < nb ? b0 + per : nb;
+    if (t == 0) gbase[0] = 0;
+    for (int grp = 0; grp < 4; ++grp) {
+        int sum = 0;
+        for (int b = b0; b < b1; ++b) sum += cnt[4 * b + grp];
+        s[t] = sum;
+        __syncthreads();
+        for (int d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan of the runs' sums
+            const int v = t >= d ? s[t - d] : 0;
+            __syncthreads();
+            s[t] += v;
+            __syncthreads();
+        }
+        if (t == 1023) gbase[grp + 1] = gbase[grp] + s[1023];
+        __syncthreads();
+        int run = gbase[grp] + s[t] - sum;
+        for (int b = b0; b < b1; ++b) {
+            off[4 * b + grp] = run;
+            run += cnt[4 * b + grp];
+        }
+        __syncthreads();
+    }
+    if (t == 0) *n = gbase[4];
+}
+
+__global__ __launch_bounds__(256) void k_list_write(const unsigned char* __restrict__ m, long F,
+                                                    const int* __restrict__ off, unsigned* __restrict__ list,
+                                                    unsigned* __restrict__ slot) {
+    __shared__ unsigned long long s[256];
+    const int t = threadIdx.x;
+    const long f0 = (long)blockIdx.x * kListFaces + 4 * t;
+    unsigned mk[4];
+    unsigned long long c = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        mk[q] = f0 + q < F ? (m[f0 + q] & 15u) : 0u;
+        c += group_counts(mk[q]);
+    }
+    s[t] = c;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const unsigned long long v = t >= d ? s[t - d] : 0ull;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    unsigned long long pos = s[t] - c;  // this thread's first position in each group, within the block
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (f0 + q >= F) break;
+        if (!mk[q]) {
+            slot[f0 + q] = ~0u;
+            continue;
+        }
+        const int grp = __popc(mk[q]) - 1;
+        const unsigned e = (unsigned)off[4 * blockIdx.x + grp] + (unsigned)((pos >> (16 * grp)) & 0xFFFFull);
+        pos += 1ull << (16 * grp);
+        slot[f0 + q] = e;
+        list[e] = (unsigned)(f0 + q);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_fill_u8(unsigned char* __restrict__ p, long n, unsigned char v) {
@@ -509,10 +609,20 @@ hipError_t launch_harvest(int dir, const double* U, const Geom& g, float* X, uns
     return hipGetLastError();
 }
 
-hipError_t launch_dilate(const unsigned char* in, unsigned char* out, int rows, int cols, int rad, bool group,
-                         hipStream_t s) {
-    const long waves = (long)rows * ((cols + 63) / 64);
-    hipLaunchKernelGGL(k_dilate, dim3((waves + 3) / 4), dim3(256), 0, s, in, out, rows, cols, rad, group ? 1 : 0);
+hipError_t launch_dilate(const unsigned char* in, unsigned char* out, int rows, int cols, int rad, hipStream_t s) {
+    const long n = (long)rows * cols;
+    hipLaunchKernelGGL(k_dilate, dim3((n + 255) / 256), dim3(256), 0, s, in, out, rows, cols, rad);
+    return hipGetLastError();
+}
+
+long list_blocks(long F) { return (F + kListFaces - 1) / kListFaces; }
+
+hipError_t launch_build_list(const unsigned char* masks, long F, int* cnt, int* off, unsigned* list, unsigned* slot,
+                             int* n, hipStream_t s) {
+    const long nb = list_blocks(F);
+    hipLaunchKernelGGL(k_list_count, dim3(nb), dim3(256), 0, s, masks, F, cnt);
+    hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(1024), 0, s, cnt, (int)nb, off, n);
+    hipLaunchKernelGGL(k_list_write, dim3(nb), dim3(256), 0, s, masks, F, off, list, slot);
     return hipGetLastError();
 }
 

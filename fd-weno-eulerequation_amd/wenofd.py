@@ -15,8 +15,8 @@ LIB_PATH = os.environ.get("WENOFD_LIBRARY") or os.path.join(HERE, "lib", "libwen
 
 OK, E_CONFIG, E_BAD_ENUM, E_NAN_DT, E_HIP, E_ARG, E_IO = range(7)
 LAYOUT_REF, LAYOUT_SOA, LAYOUT_REF_GHOSTED = 0, 1, 2
-K_GHOST, K_FLUX_X, K_FLUX_Y, K_SWITCH, K_DT, K_COUNT = range(6)
-KERNEL_NAMES = ("ghost_fill", "flux_x_rk", "flux_y", "switch", "delta_t")
+K_GHOST, K_FLUX_X, K_FLUX_Y, K_SWITCH, K_DT, K_HYB, K_PATCH, K_COUNT = range(8)
+KERNEL_NAMES = ("ghost_fill", "flux_x_rk", "flux_y", "switch", "delta_t", "hyb_stage", "hyb_patch")
 SWITCH_OFF, SWITCH_MLP, SWITCH_LINEAR, SWITCH_WENO = range(4)
 N_PARAMS = 1441
 SWITCH_WEIGHTS = os.path.join(HERE, "switch", "switch_mlp.npz")

@@ -55,7 +55,9 @@ enum {
     WENOFD_K_FLUX_Y = 2,  /* y-face fluxes */
     WENOFD_K_SWITCH = 3,  /* switch classification (MLP inference + flag dilation) */
     WENOFD_K_DT = 4,      /* CFL time-step reduction */
-    WENOFD_K_COUNT = 5
+    WENOFD_K_HYB = 5,     /* the switch's RK stage (both directions' fluxes, rhs, update) */
+    WENOFD_K_PATCH = 6,   /* the switch's corrections of the flagged (face, field) pairs */
+    WENOFD_K_COUNT = 7
 };
 
 /* Learned troubled-stencil switch modes (wenofd_switch_set). */

@@ -62,6 +62,9 @@ def lib():
         u8 = ctypes.POINTER(ctypes.c_ubyte)
         L.oracle_linear5.restype = d
         L.oracle_linear5.argtypes = [d, d, d, d, d]
+        L.oracle_hybrid_face.argtypes = [i, i, d, d, i, _dp, _dp]
+        L.oracle_linear_face.restype = d
+        L.oracle_linear_face.argtypes = [_dp, _dp]
         L.oracle_stencil_features.argtypes = [_dp, fp]
         L.oracle_weno_z_departure.restype = d
         L.oracle_weno_z_departure.argtypes = [_dp]
@@ -131,6 +134,22 @@ N_PARAMS = 1441  # W1[32][10], b1[32], W2[32][32], b2[32], w3[32], b3
 
 def linear5(a, b, c, d, e):
     return lib().oracle_linear5(float(a), float(b), float(c), float(d), float(e))
+
+
+def hybrid_face(direction, scheme, mask, U6, gamma=1.4, zp_lambda=0.0):
+    """Hybrid face flux from the conserved states of points f-3..f+2 ([6][4])."""
+    U = np.ascontiguousarray(U6, dtype=np.float64).reshape(24)
+    out = np.empty(4)
+    lib().oracle_hybrid_face(int(direction), int(scheme), float(gamma), float(zp_lambda), int(mask), _ptr(U),
+                             _ptr(out))
+    return out
+
+
+def linear_face(fp5, fm5):
+    """The cheap flux of one component: F+ at points f-3..f+1, F- at f-2..f+2."""
+    a = np.ascontiguousarray(fp5, dtype=np.float64)
+    b = np.ascontiguousarray(fm5, dtype=np.float64)
+    return lib().oracle_linear_face(_ptr(a), _ptr(b))
 
 
 def stencil_features(w5):
@@ -265,7 +284,9 @@ class Solver:
         return X, Y
 
     def set_flags(self, flags_x, flags_y):
-        """Hybrid flux: faces with a zero flag take the upwind-5 flux. None restores."""
+        """Hybrid flux (hybrid_face): per face a mask of flagged characteristic
+        fields (bit k = field k); zero takes the cheap flux alone. None restores
+        the plain solver."""
         if flags_x is None:
             self._flags = None
             lib().oracle_set_flags(self.h, None, None)

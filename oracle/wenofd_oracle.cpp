@@ -440,14 +440,71 @@ static inline void face_samples(const double vp[5][4], const double vm[5][4], fl
     }
 }
 
-// Component-wise linear upwind-5 face flux (the switch's cheap branch).
+// Component-wise linear upwind-5 face flux (the switch's cheap branch), as
+// the sum of the six points' contributions in point order: with Fp5[s] = F+ at
+// point f-3+s and Fm5[s] = F- at point f-2+s, point f+o (o = -3..2) adds
+// t_o = c+_o F+ + c-_o F-, c+ = (2, -13, 47, 27, -3, 0) and c- = (0, -3, 27,
+// 47, -13, 2) (upwind-5 on F+ and its mirror on F-); the flux is the sum
+// times 1/60. The first contribution is assigned, the others added; a zero
+// coefficient is left out.
 template <class R>
 static inline void face_flux_linear(const R (*Fp5)[4], const R (*Fm5)[4], R* Fhat) {
     for (int r = 0; r < 4; ++r) {
-        const R hp = linear5<R>(Fp5[0][r], Fp5[1][r], Fp5[2][r], Fp5[3][r], Fp5[4][r]);
-        const R hm = linear5<R>(Fm5[4][r], Fm5[3][r], Fm5[2][r], Fm5[1][r], Fm5[0][r]);
-        Fhat[r] = hp + hm;
+        R acc = R(2.0) * Fp5[0][r];                        // o = -3
+        acc = acc + (R(-13.0) * Fp5[1][r] + R(-3.0) * Fm5[0][r]);  // o = -2
+        acc = acc + (R(47.0) * Fp5[2][r] + R(27.0) * Fm5[1][r]);   // o = -1
+        acc = acc + (R(27.0) * Fp5[3][r] + R(47.0) * Fm5[2][r]);   // o = 0
+        acc = acc + (R(-3.0) * Fp5[4][r] + R(-13.0) * Fm5[3][r]);  // o = 1
+        acc = acc + R(2.0) * Fm5[4][r];                    // o = 2
+        Fhat[r] = acc * R(1.0 / 60.0);
     }
+}
+
+// Hybrid face flux. Bit k of mask flags characteristic field k: the flux is
+// the cheap flux plus, for each flagged field in ascending k, R[:,k] (w_k -
+// l_k), where w_k is the scheme's WENO5 reconstruction of field k (the +
+// stencil plus the mirrored - stencil, as face_flux) and l_k the linear
+// upwind-5 reconstruction of the same projected stencils (WENO with omega =
+// d). With every field flagged this is face_flux up to rounding (R L = I);
+// with none it is the cheap flux. The corrections are summed first, then
+// added to the cheap flux once.
+template <class R>
+static inline void hybrid_face(int mask, const R* Ul, const R* Ur, const R (*Fp5)[4], const R (*Fm5)[4], int dir,
+                               int scheme, double gamma, double zp_lambda, R* Fhat) {
+    face_flux_linear<R>(Fp5, Fm5, Fhat);
+    if (!mask) return;
+    R ave[4], L[16], Rm[16];
+    for (int r = 0; r < 4; ++r) ave[r] = R(0.5) * (Ul[r] + Ur[r]);
+    eigen_left(ave, dir, gamma, L);
+    eigen_right(ave, dir, gamma, Rm);
+    R corr[4];
+    bool first = true;
+    for (int k = 0; k < 4; ++k) {
+        if (!(mask >> k & 1)) continue;
+        R vp[5], vm[5];
+        for (int s = 0; s < 5; ++s) {
+            R a = L[k * 4 + 0] * Fp5[s][0];
+            a += L[k * 4 + 1] * Fp5[s][1];
+            a += L[k * 4 + 2] * Fp5[s][2];
+            a += L[k * 4 + 3] * Fp5[s][3];
+            vp[s] = a;
+            R b = L[k * 4 + 0] * Fm5[s][0];
+            b += L[k * 4 + 1] * Fm5[s][1];
+            b += L[k * 4 + 2] * Fm5[s][2];
+            b += L[k * 4 + 3] * Fm5[s][3];
+            vm[s] = b;
+        }
+        const R w = weno5<R>(scheme, vp[0], vp[1], vp[2], vp[3], vp[4], zp_lambda) +
+                    weno5<R>(scheme, vm[4], vm[3], vm[2], vm[1], vm[0], zp_lambda);
+        const R l = linear5<R>(vp[0], vp[1], vp[2], vp[3], vp[4]) + linear5<R>(vm[4], vm[3], vm[2], vm[1], vm[0]);
+        const R d = w - l;
+        for (int r = 0; r < 4; ++r) {
+            const R c = Rm[r * 4 + k] * d;
+            corr[r] = first ? c : corr[r] + c;
+        }
+        first = false;
+    }
+    for (int r = 0; r < 4; ++r) Fhat[r] = Fhat[r] + corr[r];
 }
 
 // MLP 10 -> 32 -> 32 -> 1, tanh-form GELU, parameters as floats:
@@ -708,9 +765,9 @@ struct Solver {
     double dx = 0, dy = 0, zp_lambda = 0, now = 0;
     std::vector<double> xc, yc;  // node coordinates incl. ghosts
     std::vector<R> Uh, Un, rhs;
-    // Switch flags (null: every face takes face_flux): x faces [NY][NX+1]
-    // (row j-4, face f-4), y faces [NY+1][NX] (face f-4, column i-4); nonzero
-    // = characteristic WENO, zero = component-wise upwind-5.
+    // Switch masks (null: every face takes face_flux): x faces [NY][NX+1]
+    // (row j-4, face f-4), y faces [NY+1][NX] (face f-4, column i-4); bit k
+    // flags characteristic field k (hybrid_face).
     const unsigned char* flags_x = nullptr;
     const unsigned char* flags_y = nullptr;
 
@@ -834,8 +891,9 @@ struct Solver {
                         fp5[s + 2][r] = Fp[(f + s - 1) * 4 + r];
                         fm5[s + 2][r] = Fm[(f + s) * 4 + r];
                     }
-                if (flags_x && !flags_x[(size_t)(j - sy) * (NX + 1) + (f - sx)])
-                    face_flux_linear(fp5, fm5, &Fh[f * 4]);
+                if (flags_x)
+                    hybrid_face<R>(flags_x[(size_t)(j - sy) * (NX + 1) + (f - sx)] & 15, &Uh[idx(f - 1, j)],
+                                   &Uh[idx(f, j)], fp5, fm5, 0, scheme, cs.gamma, zp_lambda, &Fh[f * 4]);
                 else
                     face_flux<R>(&Uh[idx(f - 1, j)], &Uh[idx(f, j)], fp5, fm5, 0, scheme, cs.gamma, zp_lambda,
                                  &Fh[f * 4]);
@@ -852,8 +910,9 @@ struct Solver {
                         fp5[s + 2][r] = Fp[(f + s - 1) * 4 + r];
                         fm5[s + 2][r] = Fm[(f + s) * 4 + r];
                     }
-                if (flags_y && !flags_y[(size_t)(f - sy) * NX + (i - sx)])
-                    face_flux_linear(fp5, fm5, &Fh[f * 4]);
+                if (flags_y)
+                    hybrid_face<R>(flags_y[(size_t)(f - sy) * NX + (i - sx)] & 15, &Uh[idx(i, f - 1)],
+                                   &Uh[idx(i, f)], fp5, fm5, 1, scheme, cs.gamma, zp_lambda, &Fh[f * 4]);
                 else
                     face_flux<R>(&Uh[idx(i, f - 1)], &Uh[idx(i, f)], fp5, fm5, 1, scheme, cs.gamma, zp_lambda,
                                  &Fh[f * 4]);
@@ -1137,8 +1196,29 @@ void oracle_samples(OracleHandle* h, int dir, float* X, unsigned char* Y) {
     h->s.samples(dir, X, Y);
 }
 
-// Flags for the hybrid flux (arrays stay owned by the caller and must outlive
-// their use); null pointers restore the plain solver.
+// Hybrid face flux (hybrid_face) from 6 consecutive points' conserved states
+// U6[6][4] (points f-3..f+2) and a field mask.
+void oracle_hybrid_face(int dir, int scheme, double gamma, double zp_lambda, int mask, const double* U6,
+                        double* Fhat) {
+    double Fm[6][4], Fp[6][4];
+    for (int k = 0; k < 6; ++k) orc::flux_split<double>(&U6[4 * k], dir, gamma, Fm[k], Fp[k]);
+    double fp5[5][4], fm5[5][4];
+    for (int s = 0; s < 5; ++s)
+        for (int r = 0; r < 4; ++r) { fp5[s][r] = Fp[s][r]; fm5[s][r] = Fm[s + 1][r]; }
+    orc::hybrid_face<double>(mask, &U6[4 * 2], &U6[4 * 3], fp5, fm5, dir, scheme, gamma, zp_lambda, Fhat);
+}
+
+// The cheap flux of one component from split values: Fp6 = F+ at points
+// f-3..f+1 (5 values), Fm5 = F- at points f-2..f+2.
+double oracle_linear_face(const double* Fp5, const double* Fm5) {
+    double fp[5][4] = {}, fm[5][4] = {}, out[4];
+    for (int s = 0; s < 5; ++s) { fp[s][0] = Fp5[s]; fm[s][0] = Fm5[s]; }
+    orc::face_flux_linear<double>(fp, fm, out);
+    return out[0];
+}
+
+// Switch masks for the hybrid flux (arrays stay owned by the caller and must
+// outlive their use); null pointers restore the plain solver.
 void oracle_set_flags(OracleHandle* h, const unsigned char* fx, const unsigned char* fy) {
     h->s.flags_x = fx;
     h->s.flags_y = fy;
