@@ -58,6 +58,7 @@ struct wenofd_solver {
     Bc bc{};
     Tiles tiles{};
     int device = 0;
+    int cus = 0;  // compute units of the device
     hipStream_t stream = nullptr;
     // Three state buffers (4 planes incl. ghosts each): U is the current
     // state; Ua and Ub hold intermediate RK stages. Stages never copy: RK3 runs
@@ -197,6 +198,9 @@ int stage(wenofd_solver* h, double* in, const double* un, double* out, const RkC
         return rc;
     if (sw) {
         const int fp = h->opt.fp_contract;
+        const int segs = hyb_stage_segments(h->g, h->cus * 4 * hyb_waves_per_simd());
+        if (hyb_stage_blocks(h->g, segs) > (long)h->part_cap)
+            return fail(WENOFD_E_HIP, "hybrid stage: more blocks than CFL partial slots");
         if ((rc = timed(h, WENOFD_K_PATCH, [&] {
                  return WENOFD_FP(fp, launch_hyb_patch(h->opt.scheme, in, h->g, h->ent_x, h->ent_y, h->fl_x,
                                                        h->fl_y, h->n_pairs, h->corr_x, h->corr_y, h->stream));
@@ -206,10 +210,10 @@ int stage(wenofd_solver* h, double* in, const double* un, double* out, const RkC
                  return WENOFD_FP(fp, launch_hyb_stage(in, un, out, rhs_out, h->g, k, h->opt.cfl,
                                                        last_stage ? h->dt_part : nullptr,
                                                        last_stage ? h->dt_part_nan : nullptr, h->slot_x, h->slot_y,
-                                                       h->corr_x, h->corr_y, h->stream));
+                                                       h->corr_x, h->corr_y, segs, h->stream));
              })))
             return rc;
-        if (last_stage) h->dt_parts = (int)hyb_stage_blocks(h->g);
+        if (last_stage) h->dt_parts = (int)hyb_stage_blocks(h->g, segs);
         return 0;
     }
     if ((rc = timed(h, WENOFD_K_FLUX_Y, [&] {
@@ -438,7 +442,8 @@ int create_impl(const std::string& text, int device, wenofd_solver** out) {
         return code;
     };
     if ((rc = set_dev(h))) return cleanup_fail(rc);
-    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    hipError_t e = hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, h->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&h->U, h->state_doubles() * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&h->Ua, h->state_doubles() * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&h->Ub, h->state_doubles() * sizeof(double));

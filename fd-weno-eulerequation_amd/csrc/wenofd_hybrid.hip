@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <utility>
 
 #include "wenofd_index.hpp"
 #include "wenofd_kernels.hpp"
@@ -44,10 +45,10 @@ namespace WENOFD_FP_NS {
 
 constexpr int kHybCols = 64;             // loaded columns per strip = lanes
 constexpr int kHybOut = kHybCols - 6;    // output columns per strip
-constexpr int kHybRows = 16;             // output rows per block
+constexpr int kHybWaves = 2;             // resident waves per SIMD the kernel is built for
+constexpr int kHybMinRows = 8;           // fewest output rows per segment
 
 __host__ __device__ inline int hyb_strips(const Geom& g) { return (g.nx + kHybOut - 1) / kHybOut; }
-__host__ __device__ inline int hyb_segments(const Geom& g) { return (g.ny + kHybRows - 1) / kHybRows; }
 
 // The cell terms with the spacings (as the plain x sweep: the build that
 // divides by the reciprocal multiplies by the host's 1/dx and 1/dy).
@@ -227,77 +228,125 @@ __device__ __forceinline__ void load_correction(unsigned slot, const double* __r
     }
 }
 
-// One RK stage of the hybrid solver on strip (bx, by):
+// The row march of the stage kernel is unrolled kHybAhead times: row q is
+// loaded kHybAhead rows before it is split, with the Un of its cell row, each
+// row of an unrolled group into its own registers, which are reloaded in
+// place once read, so that no load in flight is ever copied (a copy of a
+// register with a load in flight waits for that load). The corrections of the
+// faces of row q + 1 are loaded first thing at row q (vector memory counts
+// complete in issue order, so nothing issued earlier is waited for early),
+// from slots loaded kHybAhead rows before that. Every load reads a valid
+// address (columns and rows clamped into the array): a load under a lane
+// condition with the other lanes' registers set otherwise would also wait for
+// the load.
+constexpr int kHybAhead = 2;
+
+// One RK stage of the hybrid solver on strip blockIdx.x, segment blockIdx.y:
 //   kRkNone:  rhs only (rhs_out)     kRk1: out = in + dt rhs
 //   kRkStage: out = (a Un + b in) + (c dt) rhs
 // slot_x / slot_y: per face (x faces [ny][nx+1], y faces [ny+1][nx]) the
 // slot of its flagged fields' corrections in corr_x / corr_y. With part set,
 // the block's CFL partial minimum of its updated cells.
-template <int H>
-__global__ __launch_bounds__(kHybCols, 2) void k_hyb_stage(const double* __restrict__ Uin, const double* Un,
-                                                             double* Uout, double* __restrict__ rhs_out, Geom g,
-                                                             RkCoef k, double cfl, double* __restrict__ part,
-                                                             int* __restrict__ part_nan,
-                                                             const unsigned* __restrict__ slot_x,
-                                                             const unsigned* __restrict__ slot_y,
-                                                             const double* __restrict__ corr_x,
-                                                             const double* __restrict__ corr_y,
-                                                             const SolveState* __restrict__ st) {
+// step(j, q0 + j) for j = 0 .. NB-1, j a compile-time constant.
+template <class F, int... J>
+__device__ __forceinline__ void each_row(F& step, int q0, std::integer_sequence<int, J...>) {
+    (step(std::integral_constant<int, J>(), q0 + J), ...);
+}
+
+// Keeps the compiler from moving the reload of a register set above the last
+// reads of its previous contents: the reload then lands in the same registers
+// and the loop carries it without a copy.
+__device__ __forceinline__ void reload_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+// Raw buffer resources (stride 0, byte range = num_records): dword 3 of the
+// gfx9-family descriptor (32-bit data format); an access at an offset past
+// the range is dropped (a store) or reads 0 (a load).
+constexpr int kBufferWord3 = 0x00020000;
+typedef unsigned U2 __attribute__((ext_vector_type(2)));
+
+template <int NB, int MODE>
+__global__ __launch_bounds__(kHybCols, kHybWaves) void k_hyb_stage(const double* __restrict__ Uin, const double* Un,
+                                                                   double* Uout, double* __restrict__ rhs_out, Geom g,
+                                                                   RkCoef k, double cfl, double* __restrict__ part,
+                                                                   int* __restrict__ part_nan,
+                                                                   const unsigned* __restrict__ slot_x,
+                                                                   const unsigned* __restrict__ slot_y,
+                                                                   const double* __restrict__ corr_x,
+                                                                   const double* __restrict__ corr_y,
+                                                                   const SolveState* __restrict__ st) {
+    static_assert(NB % 2 == 0, "the corrections alternate between two register sets");
     if (st) {
         if (!st->active) return;
         k.cdt = k.cdt * st->dt;
     }
+    __shared__ double su[4][4][kHybCols];  // U of split rows: row q in slot q & 3
     __shared__ double sxp[4][kHybCols], sxm[4][kHybCols], sfx[4][kHybCols];
     const int lane = threadIdx.x;
     const int c0 = 4 + blockIdx.x * kHybOut;  // first output column
     const int i = c0 - 3 + lane;              // this lane's column
-    const bool col_in = i < g.nx + 8;
     const bool out_col = lane >= 3 && lane < 3 + kHybOut && i < g.nx + 4;
     const bool x_face = lane >= 3 && lane <= 3 + kHybOut && i <= g.nx + 4;  // face i (points i-1, i)
-    const int r0 = 4 + blockIdx.y * H;
-    const int r1 = r0 + H < g.ny + 4 ? r0 + H : g.ny + 4;
+    // segment blockIdx.y of gridDim.y: output rows [r0, r1), the rows split evenly
+    const int r0 = 4 + (int)((long)blockIdx.y * g.ny / gridDim.y);
+    const int r1 = 4 + (int)((long)(blockIdx.y + 1) * g.ny / gridDim.y);
+    const int qa = r0 - 3, qe = r1 + 2;  // the rows split
+    const int rmax = g.ny + 7;
     const bool gh_x = k.gh.on && ghost_x_edge(g, c0, c0 + kHybOut - 1);
+    // byte offsets of this lane's point in a row (clamped into the row) and
+    // of its y face / x face in a face row
+    const unsigned lo8 = (unsigned)(i < g.nx + 7 ? i : g.nx + 7) * 8u;
+    const unsigned ly4 = (unsigned)((i < 4 ? 4 : (i > g.nx + 3 ? g.nx + 3 : i)) - 4) * 4u;
+    const unsigned lx4 = (unsigned)((i < 4 ? 4 : (i > g.nx + 4 ? g.nx + 4 : i)) - 4) * 4u;
 
-    auto load_u = [&](int row, double u[4]) {
-        if (col_in) {
-            const size_t o = (size_t)row * g.pitch + i;
+    auto lane_d = [&](const double* row_ptr) -> double {
+        return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(row_ptr) + lo8);
+    };
+    auto lane_u = [&](const unsigned* row_ptr, unsigned off) -> unsigned {
+        return *reinterpret_cast<const unsigned*>(reinterpret_cast<const char*>(row_ptr) + off);
+    };
+    auto load_u = [&](int row, double u[4]) {  // U of row (clamped)
+        row = row < 0 ? 0 : (row < rmax ? row : rmax);
+        const double* p = Uin + (size_t)row * g.pitch;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) u[r] = Uin[r * g.plane + o];
-        } else {  // a harmless state in columns past the array (never used)
-            u[0] = 1.0;
-            u[1] = u[2] = 0.0;
-            u[3] = 2.5;
+        for (int r = 0; r < 4; ++r) u[r] = lane_d(p + (size_t)r * g.plane);
+    };
+    auto load_un = [&](int row, double u[4]) {  // Un of row (clamped), RK stages only
+        if constexpr (MODE == kRkStage) {
+            row = row < r0 ? r0 : (row < r1 ? row : r1 - 1);
+            const double* p = Un + (size_t)row * g.pitch;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) u[r] = lane_d(p + (size_t)r * g.plane);
         }
     };
-    auto load_un = [&](int row, double u[4]) {
-        if (out_col && k.mode == kRkStage && row < r1) {
-            const size_t o = (size_t)row * g.pitch + i;
+    // buffer stores of w to row `row` of the output planes, dropped (offset
+    // past the row) for the lanes without `on`
+    double* const dst = MODE == kRkNone ? rhs_out : Uout;
+    auto store_row = [&](int row, bool on, const double v[4]) {
+        row = row < 0 ? 0 : row;
+        const unsigned off = on ? lo8 : 0x80000000u;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) u[r] = Un[r * g.plane + o];
-        } else {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) u[r] = 0.0;
+        for (int r = 0; r < 4; ++r) {
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                dst + (size_t)r * g.plane + (size_t)row * g.pitch, (short)0, g.pitch * 8, kBufferWord3);
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v[r]), rs, off, 0, 0);
         }
     };
-
-    auto slot_of_y = [&](int face_row) {  // face_row in [r0, r1]
-        return out_col ? slot_y[(size_t)(face_row - 4) * g.nx + (i - 4)] : ~0u;
+    // raw slots of the faces of row q: y face q-2 and the x faces of cell row q-3 (clamped)
+    auto load_slots = [&](int q, unsigned& sy, unsigned& sx) {
+        int fr = q - 2, cr = q - 3;
+        fr = fr < 4 ? 4 : (fr > g.ny + 4 ? g.ny + 4 : fr);
+        cr = cr < 4 ? 4 : (cr > g.ny + 3 ? g.ny + 3 : cr);
+        sy = lane_u(slot_y + (size_t)(fr - 4) * g.nx, ly4);
+        sx = lane_u(slot_x + (size_t)(cr - 4) * (g.nx + 1), lx4);
     };
-    auto slot_of_x = [&](int row) {  // row in [r0, r1)
-        return x_face ? slot_x[(size_t)(row - 4) * (g.nx + 1) + (i - 4)] : ~0u;
-    };
 
+    double ub[NB][4];     // U of the rows of the group, loaded NB rows ahead
+    double unb[NB][4];    // Un of their cell rows
+    unsigned syb[NB], sxb[NB];  // raw slots of the faces of rows q + 1 (loaded NB rows before row q)
+    double cyb[2][4], cxb[2][4];  // corrections of the faces of rows q (set q & 1), where flagged
+    bool fyb_on[2] = {false, false}, fxb_on[2] = {false, false};  // which lanes have one
     double acc[5][4];   // acc[m]: partial cheap flux of face q-2+m (before row q)
     double fyb[4];      // flux of the bottom face of the next cell row
-    double u1[4], u2[4];  // prefetched U of rows q+1, q+2
-    double ucn[4], unn[4];  // prefetched U and Un of the next cell row
-    // prefetched slots and corrections: y faces fc+1 (slot + values) and fc+2
-    // (slot), x faces of cell rows c+1 (slot + values) and c+2 (slot)
-    unsigned sy1 = slot_of_y(r0), sy2 = slot_of_y(r0 + 1 <= r1 ? r0 + 1 : r1);
-    unsigned sx1 = slot_of_x(r0), sx2 = r0 + 1 < r1 ? slot_of_x(r0 + 1) : ~0u;
-    double cy1[4], cx1[4];
-    load_correction(sy1, corr_y, cy1);
-    load_correction(sx1, corr_x, cx1);
     double my_min = 1.0;
     int my_nan = 0;
 #pragma unroll
@@ -305,23 +354,40 @@ __global__ __launch_bounds__(kHybCols, 2) void k_hyb_stage(const double* __restr
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[m][r] = 0.0;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) fyb[r] = 0.0;
-    load_u(r0 - 3, u1);
-    load_u(r0 - 2, u2);
-    load_u(r0, ucn);
-    load_un(r0, unn);
-
-    for (int q = r0 - 3; q <= r1 + 2; ++q) {
-        // ---- row q: y split, partial sums of faces q-2 .. q+3 ----
-        double u[4];
+    for (int r = 0; r < 4; ++r) fyb[r] = cyb[0][r] = cyb[1][r] = cxb[0][r] = cxb[1][r] = 0.0;
+    // The march starts NB rows early, on rows whose only use is to load rows
+    // qa .. qa+NB-1 (their own sums feed faces below r0, never used), so that
+    // the loop is entered with no load in flight and its waits are those of
+    // the steady state.
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            u[r] = u1[r];
-            u1[r] = u2[r];
+    for (int j = 0; j < NB; ++j) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ub[j][r] = unb[j][r] = 0.0;
+        ub[j][0] = ub[j][3] = 1.0;  // a finite state for the early rows' splits
+        syb[j] = sxb[j] = ~0u;
+    }
+
+    auto step = [&](auto J, int q) {
+        constexpr int j = decltype(J)::value;
+        constexpr int cs = j & 1, cn = cs ^ 1;  // correction sets of rows q and q + 1 (NB is even)
+        if (q > qe) return;
+        // the corrections of the faces of row q + 1 this block owns
+        {
+            const int fr = q + 1 - 2, cr = q + 1 - 3;
+            const unsigned sy = (out_col && fr >= r0) ? syb[j] : ~0u;
+            const unsigned sx = (x_face && cr >= r0) ? sxb[j] : ~0u;
+            fyb_on[cn] = sy != ~0u;
+            fxb_on[cn] = sx != ~0u;
+            load_correction(sy, corr_y, cyb[cn]);
+            load_correction(sx, corr_x, cxb[cn]);
+            reload_fence();
+            load_slots(q + 1 + NB, syb[j], sxb[j]);
         }
-        if (q + 2 <= r1 + 2) load_u(q + 2, u2);
+        // ---- row q: y split, partial sums of faces q-2 .. q+3 ----
         double fm[4], fp[4], fyt[4];
-        flux_split<1>(u, g.gamma, fm, fp);
+        flux_split<1>(ub[j], g.gamma, fm, fp);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) su[q & 3][r][lane] = ub[j][r];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             fyt[r] = (acc[0][r] + cheap_term<2>(fp[r], fm[r])) * (1.0 / 60.0);  // face q-2 complete
@@ -331,111 +397,97 @@ __global__ __launch_bounds__(kHybCols, 2) void k_hyb_stage(const double* __restr
             acc[3][r] = acc[4][r] + cheap_term<-2>(fp[r], fm[r]);
             acc[4][r] = cheap_term<-3>(fp[r], fm[r]);
         }
+        reload_fence();
+        load_u(q + NB, ub[j]);
         const int fc = q - 2;  // the face just completed (wave-uniform)
-        if (fc < r0) continue;
-        if (sy1 != ~0u) {
+        const int c = fc - 1;  // the cell row updated at this row (when fc > r0)
+        double w[4] = {0.0, 0.0, 0.0, 0.0};  // its new state (its rhs with MODE kRkNone)
+        if (fc >= r0) {
+            if (fyb_on[cs]) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) fyt[r] = fyt[r] + cy1[r];
-        }
-        sy1 = sy2;
-        load_correction(sy1, corr_y, cy1);
-        if (fc + 2 <= r1) sy2 = slot_of_y(fc + 2);
-        if (fc == r0) {  // the bottom face of the first cell row
+                for (int r = 0; r < 4; ++r) fyt[r] = fyt[r] + cyb[cs][r];
+            }
+            if (fc > r0) {
+                // ---- cell row c: x faces, then the update ----
+                double uc[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) uc[r] = su[c & 3][r][lane];
+                double fxm[4], fxp[4];
+                flux_split<0>(uc, g.gamma, fxm, fxp);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    sxp[r][lane] = fxp[r];
+                    sxm[r][lane] = fxm[r];
+                }
+                wave_sync();
+                if (x_face) {  // face i: F+ of lanes lane-3 .. lane+1, F- of lane-2 .. lane+2
+                    const bool cx = fxb_on[cs];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        double a = cheap_term<-3>(sxp[r][lane - 3], 0.0);
+                        a = a + cheap_term<-2>(sxp[r][lane - 2], sxm[r][lane - 2]);
+                        a = a + cheap_term<-1>(sxp[r][lane - 1], sxm[r][lane - 1]);
+                        a = a + cheap_term<0>(sxp[r][lane], sxm[r][lane]);
+                        a = a + cheap_term<1>(sxp[r][lane + 1], sxm[r][lane + 1]);
+                        a = a + cheap_term<2>(0.0, sxm[r][lane + 2]);
+                        a = a * (1.0 / 60.0);
+                        sfx[r][lane] = cx ? a + cxb[cs][r] : a;
+                    }
+                }
+                wave_sync();
+                if (out_col) {
+                    double rhs[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        rhs[r] = hyb_cell_rhs(sfx[r][lane], sfx[r][lane + 1], fyb[r], fyt[r], g);
+                    if (k.gravity) {  // src/WENOFD.cpp:812-813
+                        rhs[3] = rhs[3] + uc[2];
+                        rhs[2] = rhs[2] + uc[0];
+                    }
+                    if constexpr (MODE == kRkNone) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) w[r] = rhs[r];
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            w[r] = (MODE == kRk1) ? uc[r] + k.cdt * rhs[r]
+                                                  : rk_combine(k.a, unb[j][r], k.b, uc[r], k.cdt, rhs[r]);
+                        const bool gh_y = k.gh.on && ghost_y_edge(g, c);
+                        if (gh_x || gh_y) {
+                            ghost_targets(
+                                g, k.gh.bc, i, c, gh_x, gh_y,
+                                [&](int di, int dj, int flip) {
+                                    const size_t d = (size_t)dj * g.pitch + di;
+#pragma unroll
+                                    for (int r = 0; r < 4; ++r) Uout[r * g.plane + d] = (r == flip) ? -w[r] : w[r];
+                                },
+                                [&](int di, int dj, int spec) {
+                                    const size_t d = (size_t)dj * g.pitch + di;
+                                    const int ns = 2 * (g.nx + g.ny);
+#pragma unroll
+                                    for (int r = 0; r < 4; ++r) Uout[r * g.plane + d] = k.gh.special[r * ns + spec];
+                                });
+                        }
+                        if (part) {
+                            const double tl = hyb_local_dt(w, g, cfl);
+                            if (tl != tl) my_nan = 1;
+                            my_min = (tl < my_min) ? tl : my_min;
+                        }
+                    }
+                }
+            }
 #pragma unroll
             for (int r = 0; r < 4; ++r) fyb[r] = fyt[r];
-            continue;
         }
+        // cell row c out (every row issues these stores: lanes and rows that
+        // own no cell are dropped by the buffer's range check)
+        store_row(c, out_col && fc > r0, w);
+        // the set now takes row q + NB
+        reload_fence();
+        load_un(q + NB - 3, unb[j]);
+    };
 
-        // ---- cell row c = fc - 1 (= q - 3): x faces, then the update ----
-        const int c = fc - 1;
-        double uc[4], un[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            uc[r] = ucn[r];
-            un[r] = unn[r];
-        }
-        if (c + 1 < r1) {
-            load_u(c + 1, ucn);
-            load_un(c + 1, unn);
-        }
-        double fxm[4], fxp[4];
-        flux_split<0>(uc, g.gamma, fxm, fxp);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            sxp[r][lane] = fxp[r];
-            sxm[r][lane] = fxm[r];
-        }
-        wave_sync();
-        if (x_face) {  // face i: F+ of lanes lane-3 .. lane+1, F- of lane-2 .. lane+2
-            double fx[4];
-#pragma unroll 1
-            for (int r = 0; r < 4; ++r) {
-                double a = cheap_term<-3>(sxp[r][lane - 3], 0.0);
-                a = a + cheap_term<-2>(sxp[r][lane - 2], sxm[r][lane - 2]);
-                a = a + cheap_term<-1>(sxp[r][lane - 1], sxm[r][lane - 1]);
-                a = a + cheap_term<0>(sxp[r][lane], sxm[r][lane]);
-                a = a + cheap_term<1>(sxp[r][lane + 1], sxm[r][lane + 1]);
-                a = a + cheap_term<2>(0.0, sxm[r][lane + 2]);
-                fx[r] = a * (1.0 / 60.0);
-            }
-            if (sx1 != ~0u) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) fx[r] = fx[r] + cx1[r];
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r) sfx[r][lane] = fx[r];
-        }
-        sx1 = sx2;
-        load_correction(sx1, corr_x, cx1);
-        sx2 = c + 2 < r1 ? slot_of_x(c + 2) : ~0u;
-        wave_sync();
-        if (out_col) {
-            double rhs[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) rhs[r] = hyb_cell_rhs(sfx[r][lane], sfx[r][lane + 1], fyb[r], fyt[r], g);
-            if (k.gravity) {  // src/WENOFD.cpp:812-813
-                rhs[3] = rhs[3] + uc[2];
-                rhs[2] = rhs[2] + uc[0];
-            }
-            const size_t o = (size_t)c * g.pitch + i;
-            if (rhs_out) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) rhs_out[r * g.plane + o] = rhs[r];
-            }
-            if (k.mode != kRkNone) {
-                double nu[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    nu[r] = (k.mode == kRk1) ? uc[r] + k.cdt * rhs[r]
-                                             : rk_combine(k.a, un[r], k.b, uc[r], k.cdt, rhs[r]);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) Uout[r * g.plane + o] = nu[r];
-                const bool gh_y = k.gh.on && ghost_y_edge(g, c);
-                if (gh_x || gh_y) {
-                    ghost_targets(
-                        g, k.gh.bc, i, c, gh_x, gh_y,
-                        [&](int di, int dj, int flip) {
-                            const size_t d = (size_t)dj * g.pitch + di;
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) Uout[r * g.plane + d] = (r == flip) ? -nu[r] : nu[r];
-                        },
-                        [&](int di, int dj, int spec) {
-                            const size_t d = (size_t)dj * g.pitch + di;
-                            const int ns = 2 * (g.nx + g.ny);
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) Uout[r * g.plane + d] = k.gh.special[r * ns + spec];
-                        });
-                }
-                if (part) {
-                    const double tl = hyb_local_dt(nu, g, cfl);
-                    if (tl != tl) my_nan = 1;
-                    my_min = (tl < my_min) ? tl : my_min;
-                }
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) fyb[r] = fyt[r];
-    }
+    for (int q0 = qa - NB; q0 <= qe; q0 += NB) each_row(step, q0, std::make_integer_sequence<int, NB>());
     if (part) {
         my_min = hyb_wave_min(my_min);
         my_nan = __any(my_nan) ? 1 : 0;
@@ -447,7 +499,19 @@ __global__ __launch_bounds__(kHybCols, 2) void k_hyb_stage(const double* __restr
     }
 }
 
-long hyb_stage_blocks(const Geom& g) { return (long)hyb_strips(g) * hyb_segments(g); }
+// Segments per strip: the grid is one round of resident_waves waves (the
+// device's CU count x 4 SIMDs x kHybWaves), so every wave marches one segment
+// and none waits for a second round; at least kHybMinRows rows per segment.
+int hyb_stage_segments(const Geom& g, int resident_waves) {
+    const int per = resident_waves / hyb_strips(g);
+    const int most = g.ny / kHybMinRows;
+    const int segs = per < most ? per : most;
+    return segs > 1 ? segs : 1;
+}
+
+long hyb_stage_blocks(const Geom& g, int segs) { return (long)hyb_strips(g) * segs; }
+
+int hyb_waves_per_simd() { return kHybWaves; }
 
 hipError_t launch_hyb_patch(int scheme, const double* Uin, const Geom& g, const unsigned* list_x,
                             const unsigned* list_y, const unsigned char* mx, const unsigned char* my,
@@ -467,11 +531,16 @@ hipError_t launch_hyb_patch(int scheme, const double* Uin, const Geom& g, const 
 
 hipError_t launch_hyb_stage(const double* Uin, const double* Un, double* Uout, double* rhs_out, const Geom& g,
                             const RkCoef& k, double cfl, double* dt_part, int* dt_nan, const unsigned* slot_x,
-                            const unsigned* slot_y, const double* corr_x, const double* corr_y, hipStream_t s,
-                            const SolveState* st) {
-    const dim3 grid(hyb_strips(g), hyb_segments(g));
-    hipLaunchKernelGGL((k_hyb_stage<kHybRows>), grid, dim3(kHybCols), 0, s, Uin, Un, Uout, rhs_out, g, k, cfl,
-                       dt_part, dt_nan, slot_x, slot_y, corr_x, corr_y, st);
+                            const unsigned* slot_y, const double* corr_x, const double* corr_y, int segs,
+                            hipStream_t s, const SolveState* st) {
+    const dim3 grid(hyb_strips(g), segs);
+    auto go = [&](auto M) {
+        hipLaunchKernelGGL((k_hyb_stage<kHybAhead, decltype(M)::value>), grid, dim3(kHybCols), 0, s, Uin, Un, Uout,
+                           rhs_out, g, k, cfl, dt_part, dt_nan, slot_x, slot_y, corr_x, corr_y, st);
+    };
+    if (k.mode == kRkNone) go(std::integral_constant<int, kRkNone>());
+    else if (k.mode == kRk1) go(std::integral_constant<int, kRk1>());
+    else go(std::integral_constant<int, kRkStage>());
     return hipGetLastError();
 }
 

@@ -124,6 +124,8 @@ using exact::launch_copy_interior;
 using exact::launch_add_source;
 using exact::flux_x_rk_blocks;
 using exact::hyb_stage_blocks;
+using exact::hyb_stage_segments;
+using exact::hyb_waves_per_simd;
 using exact::flux_y_blocks;
 using exact::ghost_blocks;
 using exact::launch_dt_final;
