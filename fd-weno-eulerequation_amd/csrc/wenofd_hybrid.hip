@@ -44,7 +44,8 @@ namespace wenofd {
 namespace WENOFD_FP_NS {
 
 constexpr int kHybCols = 64;             // loaded columns per strip = lanes
-constexpr int kHybOut = kHybCols - 6;    // output columns per strip
+constexpr int kHybLead = 5;              // lanes left of the first output column
+constexpr int kHybOut = kHybCols - 8;    // output columns per strip (lanes 5 .. 60)
 constexpr int kHybWaves = 2;             // resident waves per SIMD the kernel is built for
 constexpr int kHybMinRows = 8;           // fewest output rows per segment
 
@@ -115,15 +116,18 @@ __device__ __forceinline__ double cheap_term(double fp, double fm) {
     else return 2.0 * fm;
 }
 
-// Orders this wave's LDS accesses around an exchange between its lanes. The
-// stage kernel's blocks are single waves, whose LDS operations complete in
-// issue order, so only the compiler must not move accesses across it: no
-// barrier, and no wait for outstanding global loads (which a workgroup fence
-// would drain, serialising the row march's prefetches).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// Whole-wave lane shifts of a double (DPP wave_shr:1 / wave_shl:1, every
+// lane active): lane l receives lane l-1's value (lane 0 keeps its own), or
+// lane l+1's (lane 63 keeps its own).
+__device__ __forceinline__ double lane_from_left(double v) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xF, 0xF, false),
+                            __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double lane_from_right(double v) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xF, 0xF, false),
+                            __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xF, 0xF, false));
 }
 
 // ---------------------------------------------------------------------------
@@ -280,12 +284,12 @@ __global__ __launch_bounds__(kHybCols, kHybWaves) void k_hyb_stage(const double*
         k.cdt = k.cdt * st->dt;
     }
     __shared__ double su[4][4][kHybCols];  // U of split rows: row q in slot q & 3
-    __shared__ double sxp[4][kHybCols], sxm[4][kHybCols], sfx[4][kHybCols];
     const int lane = threadIdx.x;
     const int c0 = 4 + blockIdx.x * kHybOut;  // first output column
-    const int i = c0 - 3 + lane;              // this lane's column
-    const bool out_col = lane >= 3 && lane < 3 + kHybOut && i < g.nx + 4;
-    const bool x_face = lane >= 3 && lane <= 3 + kHybOut && i <= g.nx + 4;  // face i (points i-1, i)
+    const int i = c0 - kHybLead + lane;       // this lane's column
+    const int lx2 = lane + 2 < kHybCols ? lane + 2 : kHybCols - 1;  // the lane two columns right
+    const bool out_col = lane >= kHybLead && lane < kHybLead + kHybOut && i < g.nx + 4;
+    const bool x_face = lane >= kHybLead && lane <= kHybLead + kHybOut && i <= g.nx + 4;  // face i (points i-1, i)
     // segment blockIdx.y of gridDim.y: output rows [r0, r1), the rows split evenly
     const int r0 = 4 + (int)((long)blockIdx.y * g.ny / gridDim.y);
     const int r1 = 4 + (int)((long)(blockIdx.y + 1) * g.ny / gridDim.y);
@@ -294,7 +298,7 @@ __global__ __launch_bounds__(kHybCols, kHybWaves) void k_hyb_stage(const double*
     const bool gh_x = k.gh.on && ghost_x_edge(g, c0, c0 + kHybOut - 1);
     // byte offsets of this lane's point in a row (clamped into the row) and
     // of its y face / x face in a face row
-    const unsigned lo8 = (unsigned)(i < g.nx + 7 ? i : g.nx + 7) * 8u;
+    const unsigned lo8 = (unsigned)(i < 0 ? 0 : (i < g.nx + 7 ? i : g.nx + 7)) * 8u;
     const unsigned ly4 = (unsigned)((i < 4 ? 4 : (i > g.nx + 3 ? g.nx + 3 : i)) - 4) * 4u;
     const unsigned lx4 = (unsigned)((i < 4 ? 4 : (i > g.nx + 4 ? g.nx + 4 : i)) - 4) * 4u;
 
@@ -409,37 +413,36 @@ __global__ __launch_bounds__(kHybCols, kHybWaves) void k_hyb_stage(const double*
             }
             if (fc > r0) {
                 // ---- cell row c: x faces, then the update ----
-                double uc[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) uc[r] = su[c & 3][r][lane];
-                double fxm[4], fxp[4];
-                flux_split<0>(uc, g.gamma, fxm, fxp);
+                // Face i (points i-1, i) sums the cheap-flux terms of points
+                // i-3 .. i+2 in that order: each lane splits the point two
+                // columns to its right, and the partial sum moves one lane to
+                // the right before each next term is added, so lane l ends
+                // with its own column's left face (lanes 5 .. 61 whole).
+                double uc[4], ux[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    sxp[r][lane] = fxp[r];
-                    sxm[r][lane] = fxm[r];
+                    uc[r] = su[c & 3][r][lane];
+                    ux[r] = su[c & 3][r][lx2];
                 }
-                wave_sync();
-                if (x_face) {  // face i: F+ of lanes lane-3 .. lane+1, F- of lane-2 .. lane+2
-                    const bool cx = fxb_on[cs];
+                double fxm[4], fxp[4], fl[4], fr[4];
+                flux_split<0>(ux, g.gamma, fxm, fxp);
+                const bool cx = fxb_on[cs];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        double a = cheap_term<-3>(sxp[r][lane - 3], 0.0);
-                        a = a + cheap_term<-2>(sxp[r][lane - 2], sxm[r][lane - 2]);
-                        a = a + cheap_term<-1>(sxp[r][lane - 1], sxm[r][lane - 1]);
-                        a = a + cheap_term<0>(sxp[r][lane], sxm[r][lane]);
-                        a = a + cheap_term<1>(sxp[r][lane + 1], sxm[r][lane + 1]);
-                        a = a + cheap_term<2>(0.0, sxm[r][lane + 2]);
-                        a = a * (1.0 / 60.0);
-                        sfx[r][lane] = cx ? a + cxb[cs][r] : a;
-                    }
+                for (int r = 0; r < 4; ++r) {
+                    double a = cheap_term<-3>(fxp[r], 0.0);
+                    a = lane_from_left(a) + cheap_term<-2>(fxp[r], fxm[r]);
+                    a = lane_from_left(a) + cheap_term<-1>(fxp[r], fxm[r]);
+                    a = lane_from_left(a) + cheap_term<0>(fxp[r], fxm[r]);
+                    a = lane_from_left(a) + cheap_term<1>(fxp[r], fxm[r]);
+                    a = lane_from_left(a) + cheap_term<2>(0.0, fxm[r]);
+                    a = a * (1.0 / 60.0);
+                    fl[r] = cx ? a + cxb[cs][r] : a;
+                    fr[r] = lane_from_right(fl[r]);  // the right face: the next lane's left face
                 }
-                wave_sync();
                 if (out_col) {
                     double rhs[4];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        rhs[r] = hyb_cell_rhs(sfx[r][lane], sfx[r][lane + 1], fyb[r], fyt[r], g);
+                    for (int r = 0; r < 4; ++r) rhs[r] = hyb_cell_rhs(fl[r], fr[r], fyb[r], fyt[r], g);
                     if (k.gravity) {  // src/WENOFD.cpp:812-813
                         rhs[3] = rhs[3] + uc[2];
                         rhs[2] = rhs[2] + uc[0];
