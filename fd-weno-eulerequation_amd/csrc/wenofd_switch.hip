@@ -21,8 +21,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // MLP inference on the matrix cores, transposed so that one layer's output
 // registers are the next layer's B operand without any data exchange:
 //   H1^T (32 x 16 samples) = W1 (32 x 16 features, 10 used) . X^T
-//   H2^T (32 x 16)         = W2 (32 x 32) . gelu(H1^T + b1)
-//   z (1 x 16)             = w3 . gelu(H2^T + b2) + b3   (VALU + 2 shuffles)
+//   H2^T (32 x 16)         = W2 (32 x 32) . relu(H1^T + b1)
+//   z (1 x 16)             = w3 . relu(H2^T + b2) + b3   (VALU + 2 shuffles)
 // v_mfma_f32_16x16x16_bf16 lane layouts (lane l, j = 0..3):
 //   A[m = l%16][k = 4(l/16)+j], B[k = 4(l/16)+j][n = l%16], D[m = 4(l/16)+j][n = l%16].
 // One call handles 16 samples: lane l supplies features 4(l/16)..+3 of sample
@@ -62,7 +62,7 @@ __device__ __forceinline__ float mlp_group(const MlpRegs& R, bf16x4 x) {
         const f32x4 b = {R.b1[m][0], R.b1[m][1], R.b1[m][2], R.b1[m][3]};
         const f32x4 c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w1[m], x, b, 0, 0, 0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) h[m][j] = (__bf16)gelu_fast(c[j]);
+        for (int j = 0; j < 4; ++j) h[m][j] = (__bf16)relu_f(c[j]);
     }
     float z = 0.0f;
 #pragma unroll
@@ -71,7 +71,7 @@ __device__ __forceinline__ float mlp_group(const MlpRegs& R, bf16x4 x) {
         f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w2[m][0], h[0], b, 0, 0, 0);
         d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w2[m][1], h[1], d, 0, 0, 0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) z += R.w3[m][j] * gelu_fast(d[j]);
+        for (int j = 0; j < 4; ++j) z += R.w3[m][j] * relu_f(d[j]);
     }
     z += __shfl_xor(z, 16, 64);
     z += __shfl_xor(z, 32, 64);
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ P, c
         float a = P[kOffB1 + u];
         for (int c = 0; c < kFeat; ++c) a += P[kOffW1 + u * kFeat + c] * sX[sl][c];
         sA1[sl][u] = a;
-        sH1[sl][u] = gelu_f(a);
+        sH1[sl][u] = relu_f(a);
     }
     __syncthreads();
     float zp = 0.0f;
@@ -500,7 +500,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ P, c
         float a = P[kOffB2 + u];
         for (int c = 0; c < kHid; ++c) a += P[kOffW2 + u * kHid + c] * sH1[sl][c];
         sA2[sl][u] = a;
-        const float h = gelu_f(a);
+        const float h = relu_f(a);
         sH2[sl][u] = h;
         zp += P[kOffW3 + u] * h;
     }
@@ -516,12 +516,12 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ P, c
         sLoss[sl] = valid ? (pw * y * spm + (1.0f - y) * spz) : 0.0f;
         sDz[sl] = dz;
     }
-    for (int u = 8 * qq; u < 8 * qq + 8; ++u) sD2[sl][u] = dz * P[kOffW3 + u] * gelu_grad_f(sA2[sl][u]);
+    for (int u = 8 * qq; u < 8 * qq + 8; ++u) sD2[sl][u] = dz * P[kOffW3 + u] * relu_grad_f(sA2[sl][u]);
     __syncthreads();
     for (int u = 8 * qq; u < 8 * qq + 8; ++u) {
         float d = 0.0f;
         for (int c = 0; c < kHid; ++c) d += P[kOffW2 + c * kHid + u] * sD2[sl][c];
-        sD1[sl][u] = d * gelu_grad_f(sA1[sl][u]);
+        sD1[sl][u] = d * relu_grad_f(sA1[sl][u]);
     }
     __syncthreads();
     float* out = partial + (size_t)blockIdx.x * kNParams;

@@ -14,7 +14,7 @@
 //   v+_k[0..4] and the mirrored - stencil v-_k[4..0], each centred on its mean
 //   and scaled by 1 / (max |deviation| + 1e-10), rounded to float
 //   label: max over both sides and j of |omega_j - d_j| > 0.05 (WENO-Z weights)
-//   MLP 10 -> 32 -> 32 -> 1 with tanh-form GELU, one logit per (face, field);
+//   MLP 10 -> 32 -> 32 -> 1 with ReLU, one logit per (face, field);
 //   a face is flagged when any of its 4 fields has a logit above the threshold.
 #pragma once
 
@@ -36,8 +36,6 @@ constexpr int kOffB3 = kOffW3 + kHid;          // 1440
 constexpr int kNParams = kOffB3 + 1;           // 1441
 constexpr double kLabelThreshold = 0.05;
 constexpr double kFeatScaleEps = 1e-10;
-constexpr float kGeluK0 = 0.7978845608028654f;  // sqrt(2/pi)
-constexpr float kGeluK1 = 0.044715f;
 
 template <class T>
 WENOFD_HD T linear5(T a, T b, T c, T d, T e) {
@@ -118,23 +116,8 @@ WENOFD_HD void face_flux_linear(GetP sp, GetM sm, T Fh[4]) {
     }
 }
 
-// GELU, tanh form: 0.5 x (1 + tanh(k0 (x + k1 x^3))), and its derivative.
-__device__ __forceinline__ float gelu_f(float x) {
-    const float t = tanhf(kGeluK0 * (x + kGeluK1 * x * x * x));
-    return 0.5f * x * (1.0f + t);
-}
-__device__ __forceinline__ float gelu_grad_f(float x) {
-    const float x2 = x * x;
-    const float t = tanhf(kGeluK0 * (x + kGeluK1 * x2 * x));
-    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluK0 * (1.0f + 3.0f * kGeluK1 * x2);
-}
-// The same function as x * sigmoid(2 k0 (x + k1 x^3)) with the hardware
-// exp2 and reciprocal (inference).
-__device__ __forceinline__ float gelu_fast(float x) {
-    constexpr float c0 = -2.0f * kGeluK0 * 1.4426950408889634f;
-    constexpr float c1 = c0 * kGeluK1;
-    const float y = x * (c1 * (x * x) + c0);
-    return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(y));
-}
+// The hidden activation, ReLU, and its derivative (0 at the kink).
+__device__ __forceinline__ float relu_f(float x) { return x > 0.0f ? x : 0.0f; }
+__device__ __forceinline__ float relu_grad_f(float x) { return x > 0.0f ? 1.0f : 0.0f; }
 
 }  // namespace wenofd

@@ -205,17 +205,19 @@ int wenofd_synchronize(wenofd_solver* h);
 int wenofd_set_tiles(wenofd_solver* h, int x_faces, int y_faces);
 
 /*
- * Learned troubled-stencil switch. No counterpart in the reference. At the
- * start of every step the switch MLP (10 -> 32 -> 32 -> 1, GELU; bf16 MFMA
+ * Learned troubled-stencil switch. No counterpart in the reference. Every
+ * `interval` steps the switch MLP (10 -> 32 -> 32 -> 1, ReLU; bf16 MFMA
  * inference, FP32 accumulate) reads the normalised 5-point stencils of each
- * characteristic field of each face and flags the faces whose WENO-Z weights
- * it predicts to leave the linear weights (logit > threshold for any field).
- * The flags are dilated by one face (the y faces' flags are then shared by
- * each aligned group of 64 columns of a face row, the faces one wave of the y
- * sweep computes) and serve all stages of the step: flagged
- * faces take the reference's characteristic WENO flux, the others the cheap
- * component-wise linear upwind-5 flux. The plain solver is the fallback: a
- * switched step that leads to a NaN time step is redone without the switch.
+ * characteristic field of each face and flags the fields whose WENO-Z weights
+ * it predicts to leave the linear weights (logit > threshold): a 4-bit mask
+ * per face, bit k for field k. The masks are dilated (each field's flags on
+ * their own, a square of `radius` faces) and serve every stage until the next
+ * classification. Each face's flux is the cheap component-wise linear
+ * upwind-5 flux plus, for each flagged field k, R[:,k] (w_k - l_k): the
+ * characteristic WENO reconstruction of field k minus its linear one. With
+ * every field flagged this is the reference's characteristic WENO flux up to
+ * rounding. The plain solver is the fallback: a switched step that leads to a
+ * NaN time step is redone without the switch.
  * params: 1441 floats, W1[32][10], b1[32], W2[32][32], b2[32], w3[32], b3.
  */
 int wenofd_switch_set(wenofd_solver* h, int mode, const float* params, float threshold);

@@ -507,16 +507,12 @@ static inline void hybrid_face(int mask, const R* Ul, const R* Ur, const R (*Fp5
     for (int r = 0; r < 4; ++r) Fhat[r] = Fhat[r] + corr[r];
 }
 
-// MLP 10 -> 32 -> 32 -> 1, tanh-form GELU, parameters as floats:
+// MLP 10 -> 32 -> 32 -> 1, ReLU, parameters as floats:
 // W1[32][10], b1[32], W2[32][32], b2[32], w3[32], b3.
 enum { P_W1 = 0, P_B1 = 320, P_W2 = 352, P_B2 = 1376, P_W3 = 1408, P_B3 = 1440, P_N = 1441 };
-static const double kGk0 = (double)0.7978845608028654f, kGk1 = (double)0.044715f;
 
-static inline double gelu(double x) { return 0.5 * x * (1.0 + std::tanh(kGk0 * (x + kGk1 * x * x * x))); }
-static inline double gelu_grad(double x) {
-    const double t = std::tanh(kGk0 * (x + kGk1 * x * x * x));
-    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * kGk0 * (1.0 + 3.0 * kGk1 * x * x);
-}
+static inline double relu(double x) { return x > 0.0 ? x : 0.0; }
+static inline double relu_grad(double x) { return x > 0.0 ? 1.0 : 0.0; }
 
 struct MlpCache {
     double a1[32], h1[32], a2[32], h2[32], z;
@@ -527,13 +523,13 @@ static inline double mlp_forward(const double* P, const float* x, MlpCache& c) {
         double a = P[P_B1 + u];
         for (int k = 0; k < 10; ++k) a += P[P_W1 + u * 10 + k] * (double)x[k];
         c.a1[u] = a;
-        c.h1[u] = gelu(a);
+        c.h1[u] = relu(a);
     }
     for (int u = 0; u < 32; ++u) {
         double a = P[P_B2 + u];
         for (int k = 0; k < 32; ++k) a += P[P_W2 + u * 32 + k] * c.h1[k];
         c.a2[u] = a;
-        c.h2[u] = gelu(a);
+        c.h2[u] = relu(a);
     }
     double z = P[P_B3];
     for (int k = 0; k < 32; ++k) z += P[P_W3 + k] * c.h2[k];
@@ -559,7 +555,7 @@ static double mlp_loss_grad(const double* P, const float* X, const unsigned char
         G[P_B3] += dz;
         for (int k = 0; k < 32; ++k) {
             G[P_W3 + k] += dz * c.h2[k];
-            da2[k] = dz * P[P_W3 + k] * gelu_grad(c.a2[k]);
+            da2[k] = dz * P[P_W3 + k] * relu_grad(c.a2[k]);
         }
         for (int u = 0; u < 32; ++u) {
             G[P_B2 + u] += da2[u];
@@ -569,7 +565,7 @@ static double mlp_loss_grad(const double* P, const float* X, const unsigned char
             }
         }
         for (int u = 0; u < 32; ++u) {
-            const double da1 = dh1[u] * gelu_grad(c.a1[u]);
+            const double da1 = dh1[u] * relu_grad(c.a1[u]);
             G[P_B1 + u] += da1;
             for (int k = 0; k < 10; ++k) G[P_W1 + u * 10 + k] += da1 * (double)X[10 * (size_t)s + k];
         }
@@ -598,13 +594,13 @@ static inline float mlp_infer_bf16(const float* P, const float* x) {
         float a = 0;
         for (int k = 0; k < 10; ++k) a += bf16_round(P[P_W1 + u * 10 + k]) * xb[k];
         a += P[P_B1 + u];
-        h1[u] = bf16_round((float)gelu((double)a));
+        h1[u] = bf16_round((float)relu((double)a));
     }
     for (int u = 0; u < 32; ++u) {
         float a = 0;
         for (int k = 0; k < 32; ++k) a += bf16_round(P[P_W2 + u * 32 + k]) * h1[k];
         a += P[P_B2 + u];
-        h2[u] = (float)gelu((double)a);
+        h2[u] = (float)relu((double)a);
     }
     float z = 0;
     for (int k = 0; k < 32; ++k) z += P[P_W3 + k] * h2[k];

@@ -192,7 +192,8 @@ def test_fused_switched_steps_match_oracle_with_the_same_flags(mode, scheme):
             dt = g.step()
             fx, fy = g.switch_flags()
             if mode == "mlp":
-                assert 0 < fx.mean() < 1 and 0 < fy.mean() < 1
+                # some fields flagged, and some not: both the correction and the cheap flux alone run
+                assert (fx != 0).any() and (fx != 15).any() and (fy != 0).any() and (fy != 15).any()
             o.set_flags(fx, fy)
             o.step(dt)
             assert relmax(g.get_state(), o.get_state()) <= 1e-13

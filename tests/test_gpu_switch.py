@@ -55,6 +55,23 @@ def random_params(seed, scale=1.0):
     return P
 
 
+def top_threshold(lx, ly, frac=0.005):
+    """A logit threshold that flags about frac of the faces of each direction
+    (the lower of the two directions' quantiles). Faces of a constant region
+    share one logit, so a quantile may sit on a tie that flags nothing: then
+    the next distinct value below it is taken."""
+    thr = []
+    for lg in (lx, ly):
+        v = lg.max(axis=2).ravel()
+        t = float(np.quantile(v, 1.0 - frac))
+        if not (v > t).any():
+            below = np.unique(v[v < t])
+            assert below.size, "every face has the same logit"
+            t = float(below[-1])
+        thr.append(t)
+    return min(thr)
+
+
 @pytest.mark.parametrize("tc,seed", [(5, 7), (4, None), (3, None)])
 def test_harvest_matches_oracle(tc, seed):
     g, o = pair(tc, 22, 19, seed=seed)
@@ -193,8 +210,7 @@ def test_switched_steps_match_oracle_with_the_same_flags(rk):
     P = random_params(4, scale=2.0)
     g.switch_set(W.SWITCH_MLP, P, 0.0)
     _, _, lx, ly = g.switch_classify(dilate=False, logits=True)
-    # a threshold that leaves about 0.5 % of the faces flagged before dilation
-    thr = float(np.quantile(np.concatenate([lx.max(axis=2).ravel(), ly.max(axis=2).ravel()]), 0.995))
+    thr = top_threshold(lx, ly)  # about 0.5 % of the faces flagged before dilation
     g.switch_set(W.SWITCH_MLP, P, thr)
     mixed_x, mixed_y = [], []
     for _ in range(3):
@@ -214,12 +230,12 @@ def test_switched_steps_match_oracle_with_the_same_flags(rk):
 def test_switched_steps_of_every_boundary_kind_match_oracle(tc, nx, ny):
     """Switched RK3 steps on periodic (SMOOTH, VORTEX) and slip + SPECIAL
     (RTI) boundaries equal the oracle's hybrid steps bit for bit, given the
-    same flags (random network, threshold at the 99.5 % logit quantile)."""
+    same flags (random network, threshold flagging about 0.5 % of the faces)."""
     g, o = pair(tc, nx, ny, rk=2)
     P = random_params(6, scale=2.0)
     g.switch_set(W.SWITCH_MLP, P, 0.0)
     _, _, lx, ly = g.switch_classify(dilate=False, logits=True)
-    thr = float(np.quantile(np.concatenate([lx.max(axis=2).ravel(), ly.max(axis=2).ravel()]), 0.995))
+    thr = top_threshold(lx, ly)
     g.switch_set(W.SWITCH_MLP, P, thr)
     mixed_x, mixed_y = [], []
     for _ in range(3):
@@ -338,7 +354,7 @@ def test_adam_steps_follow_kingma_ba():
         P2 = m.params.astype(np.float64)
     g1, g2 = g1.astype(np.float64), g2.astype(np.float64)
     big = np.abs(g1) > 1e-4
-    assert big.sum() > 1000
+    assert big.sum() > 600  # ReLU units that no sample activates have zero gradient
     step1 = P1 - P0
     assert np.all(np.abs(step1) <= lr * (1 + 1e-5))
     assert np.all(np.abs(step1[big] + lr * np.sign(g1[big])) <= 2e-3 * lr)
