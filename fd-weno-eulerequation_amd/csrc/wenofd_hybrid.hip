@@ -197,6 +197,31 @@ __device__ __forceinline__ void patch_face(const double* __restrict__ Uin, const
     for (int r = 0; r < 4; ++r) out[r] = corr[r];
 }
 
+// List entries e0, e0 + stride, ... below n of one direction. The next
+// entry's face is loaded while the current one is computed, and its mask
+// right after (needed only once its points are in); entries alternate
+// between two register sets, reloaded in place, and every load reads a valid
+// entry (clamped), so no load in flight is waited for early.
+template <int S, int D>
+__device__ __forceinline__ void patch_run(const double* __restrict__ Uin, const Geom& g,
+                                          const unsigned* __restrict__ list, const unsigned char* __restrict__ masks,
+                                          int n, double* __restrict__ corr, int e0, int stride) {
+    if (e0 >= n) return;
+    auto entry = [&](int e) { return list[e < n ? e : n - 1]; };
+    unsigned fa = entry(e0);
+    unsigned ma = masks[fa];
+    for (int e = e0; e < n; e += 2 * stride) {
+        const int eb = e + stride;
+        unsigned fb = entry(eb);
+        patch_face<S, D>(Uin, g, fa, ma & 15u, corr + (size_t)e * 4);
+        unsigned mb = masks[fb];
+        if (eb >= n) break;
+        fa = entry(eb + stride);
+        patch_face<S, D>(Uin, g, fb, mb & 15u, corr + (size_t)eb * 4);
+        ma = masks[fa];
+    }
+}
+
 template <int S>
 __global__ __launch_bounds__(kPatchThreads) void k_hyb_patch(const double* __restrict__ Uin, Geom g,
                                                              const unsigned* __restrict__ list_x,
@@ -212,15 +237,9 @@ __global__ __launch_bounds__(kPatchThreads) void k_hyb_patch(const double* __res
     const bool xdir = (int)blockIdx.x < nb;
     const int b = xdir ? blockIdx.x : blockIdx.x - nb;
     const int n = n_dev[xdir ? 0 : 1];
-    for (int e = b * kPatchThreads + threadIdx.x; e < n; e += nb * kPatchThreads) {
-        if (xdir) {
-            const unsigned face = list_x[e];
-            patch_face<S, 0>(Uin, g, face, mx[face] & 15u, corr_x + (size_t)e * 4);
-        } else {
-            const unsigned face = list_y[e];
-            patch_face<S, 1>(Uin, g, face, my[face] & 15u, corr_y + (size_t)e * 4);
-        }
-    }
+    const int e0 = b * kPatchThreads + threadIdx.x, stride = nb * kPatchThreads;
+    if (xdir) patch_run<S, 0>(Uin, g, list_x, mx, n, corr_x, e0, stride);
+    else patch_run<S, 1>(Uin, g, list_y, my, n, corr_y, e0, stride);
 }
 
 // The summed correction of a face's flagged fields (loaded when its slot is
