@@ -250,15 +250,17 @@ def batched(W, steps, seq_steps, fp, seed=0):
     return out
 
 
-HYBRID_CONFIGS = ((1, 1), (8, 2), (16, 3))  # (classification interval, dilation radius)
+HYBRID_CONFIGS = ((8, 1), (16, 1), (16, 2))  # (classification interval, dilation radius)
 
 
 def hybrid(W, n, seed, t_end, truth, fp):
     """Learned switch against plain WENO-Z on an unseen seeded four-quadrant
     problem (seed outside the training seeds 0..255), RK3 CFL 0.6 to t_end,
-    each run timed whole: classification every step with 3 x 3 dilation, every
-    8 steps with 5 x 5 and every 16 steps with 7 x 7. With truth, all are also compared with
-    plain WENO-Z on the nested (2n-1)^2 grid sampled at even indices."""
+    each run timed whole, for each (classification interval, dilation radius)
+    of HYBRID_CONFIGS. With truth, all are also compared with plain WENO-Z on
+    the nested (2n-1)^2 grid sampled at even indices; the reported speedup is
+    the best among the configurations whose L1 error is within 5 % of the
+    plain solver's."""
     P, thr = W.load_switch()
     cfg = W.config_text(TESTCASE=5, SCHEME=1, RKMETH=2, CFL=0.6, ELEMNUMX=n, ELEMNUMY=n, QUAD_SEED=seed,
                         TEND="%.17g" % t_end, FP_CONTRACT=FP_MODES[fp])

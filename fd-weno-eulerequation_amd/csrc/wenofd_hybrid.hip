@@ -14,17 +14,20 @@
 // faces with flagged fields per thread (the list is built once per
 // classification): the face's six stencil points (L2), their split, the mean
 // state's eigen parts, then per flagged field row k of L, column k of R, the
-// projections, the WENO and linear reconstructions and the correction. k_hyb_stage then does the rest, one wave per block: a block
-// owns a strip of 64 loaded columns (58 output columns, 3 halo columns each
-// side) and a segment of H output rows; lane l owns column i = c0 - 3 + l
-// and marches up the rows.
+// projections, the WENO and linear reconstructions and the correction.
+// k_hyb_stage then does the rest, one wave per block: a block owns a strip of
+// 64 loaded columns (56 output columns, 5 lanes left and 3 right of them) and
+// a segment of rows; lane l owns column i = c0 - 5 + l and marches up the rows.
 //   y faces: each loaded row's y split is added into the partial sums of the
 //            six faces it reaches (registers; a face completes once the row
 //            two above it is in), so each point is split once per direction;
-//   x faces: the row's x split goes through wave-local LDS to the lanes of
-//            the faces that read it;
+//   x faces: from a wave-local LDS ring of the last four rows, each lane
+//            splits the point two columns to its right and the face sums
+//            travel between lanes by DPP shifts;
 //   a face with flagged fields adds their summed correction (its slot: its
-//   position in the list), prefetched a row ahead with the slots.
+//   position in the list), loaded a row ahead, its slot further ahead.
+// The grid is one round of resident waves (no tail), strips and segments
+// ordered so that each XCD takes neighbouring ones.
 // Compiled twice, like wenofd_kernels.hip: exact (no contraction) and fused.
 #include <hip/hip_runtime.h>
 
@@ -264,7 +267,7 @@ __device__ __forceinline__ void load_correction(unsigned slot, const double* __r
 // the load.
 constexpr int kHybAhead = 2;
 
-// One RK stage of the hybrid solver on strip blockIdx.x, segment blockIdx.y:
+// One RK stage of the hybrid solver on one strip of columns and segment of rows:
 //   kRkNone:  rhs only (rhs_out)     kRk1: out = in + dt rhs
 //   kRkStage: out = (a Un + b in) + (c dt) rhs
 // slot_x / slot_y: per face (x faces [ny][nx+1], y faces [ny+1][nx]) the
@@ -304,14 +307,20 @@ __global__ __launch_bounds__(kHybCols, kHybWaves) void k_hyb_stage(const double*
     }
     __shared__ double su[4][4][kHybCols];  // U of split rows: row q in slot q & 3
     const int lane = threadIdx.x;
-    const int c0 = 4 + blockIdx.x * kHybOut;  // first output column
+    // the block's strip and segment: consecutive blocks (one XCD each, in
+    // turn) map to ranges of neighbouring strips and segments per XCD, so the
+    // halo columns and rows that neighbours both read share an L2
+    const int nblk = gridDim.x * gridDim.y;
+    const int lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, nblk);
+    const int bx = lb % gridDim.x, by = lb / gridDim.x;
+    const int c0 = 4 + bx * kHybOut;  // first output column
     const int i = c0 - kHybLead + lane;       // this lane's column
     const int lx2 = lane + 2 < kHybCols ? lane + 2 : kHybCols - 1;  // the lane two columns right
     const bool out_col = lane >= kHybLead && lane < kHybLead + kHybOut && i < g.nx + 4;
     const bool x_face = lane >= kHybLead && lane <= kHybLead + kHybOut && i <= g.nx + 4;  // face i (points i-1, i)
-    // segment blockIdx.y of gridDim.y: output rows [r0, r1), the rows split evenly
-    const int r0 = 4 + (int)((long)blockIdx.y * g.ny / gridDim.y);
-    const int r1 = 4 + (int)((long)(blockIdx.y + 1) * g.ny / gridDim.y);
+    // segment by of gridDim.y: output rows [r0, r1), the rows split evenly
+    const int r0 = 4 + (int)((long)by * g.ny / gridDim.y);
+    const int r1 = 4 + (int)((long)(by + 1) * g.ny / gridDim.y);
     const int qa = r0 - 3, qe = r1 + 2;  // the rows split
     const int rmax = g.ny + 7;
     const bool gh_x = k.gh.on && ghost_x_edge(g, c0, c0 + kHybOut - 1);
