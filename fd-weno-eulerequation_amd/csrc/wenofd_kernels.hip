@@ -108,6 +108,15 @@ __global__ __launch_bounds__(kGhostThreads) void k_ghost_fill(double* __restrict
     ghost_point(U, special, g, bc, blockIdx.x * kGhostThreads + threadIdx.x);
 }
 
+// Buffer store of v at byte offset off of a row of `pitch` doubles starting at
+// row_ptr: an offset past the row (e.g. 0x80000000) drops the store (the
+// range check of a raw buffer resource; gfx9-family descriptor word 3).
+__device__ __forceinline__ void store_lane(double* row_ptr, int pitch, unsigned off, double v) {
+    typedef unsigned U2 __attribute__((ext_vector_type(2)));
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(row_ptr, (short)0, pitch * 8, 0x00020000);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), rs, off, 0, 0);
+}
+
 // ---------------------------------------------------------------------------
 // y faces. One tile = 64 interior columns x YF faces along y, marched in steps
 // of 4 face rows (256 threads = 64 columns x 4 rows). The LLF split of the 9
@@ -136,13 +145,20 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
     const int F1 = y_tile_end_face<YF>(g, by);  // one past the last face
     const int base = F0 - 3;                    // point row held in ring slot 0
 
-    auto load_row = [&](int prow) {
+    // U of point row prow in this lane's column, clamped into the array (the
+    // values of clamped rows and columns are never used)
+    const int il = i < g.nx + 7 ? i : g.nx + 7;
+    auto fetch_row = [&](int prow, double u[4]) {
+        prow = prow < g.ny + 7 ? prow : g.ny + 7;
+        const size_t o = (size_t)prow * g.pitch + il;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) u[r] = U[r * g.plane + o];
+    };
+    // the split of point row prow (its U in u) into the rings
+    auto put_row = [&](int prow, const double u[4]) {
         const int slot = (prow - base) % kRing;
         if (col_ok && prow < g.ny + 8) {
-            double u[4], fm[4], fp[4];
-            const size_t o = (size_t)prow * g.pitch + i;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) u[r] = U[r * g.plane + o];
+            double fm[4], fp[4];
             flux_split<1>(u, g.gamma, fm, fp);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -158,28 +174,46 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
             }
         }
     };
-    for (int k = ty; k < 5; k += kYR) load_row(base + k);  // lead-in: point rows F0-3 .. F0+1
+    for (int k = ty; k < 5; k += kYR) {  // lead-in: point rows F0-3 .. F0+1
+        double u[4];
+        fetch_row(base + k, u);
+        put_row(base + k, u);
+    }
+    // Each step's new row is loaded one step ahead, into registers reloaded in
+    // place, so its latency hides behind the previous step's faces; the face
+    // stores are buffer stores issued by every lane on every step (lanes and
+    // rows past the tile are dropped by the range check), so the wait for the
+    // prefetched row at the next step leaves them in flight.
+    double un[4];
+    fetch_row(F0 + 2 + ty, un);
+    // four dropped stores, as every step issues after its prefetch: the loop
+    // is then entered with the steady state's memory operations in flight
+#pragma unroll
+    for (int r = 0; r < 4; ++r) store_lane(Fy, g.fy_pitch, 0x80000000u, 0.0);
+    const unsigned lo8 = (unsigned)(i - 4) * 8u;
     for (int f = F0; f < F1; f += kYR) {
-        load_row(f + 2 + ty);  // rows f+2 .. f+5 complete the window [f-3, f+6)
+        put_row(f + 2 + ty, un);  // rows f+2 .. f+5 complete the window [f-3, f+6)
+        fetch_row(f + kYR + 2 + ty, un);
         __syncthreads();
-        auto full_face = [&](int cx, int fr) {
-            const int ic = 4 + bx * kYC + cx;
-            double Ul[4], Ur[4], Fh[4];
+        const int fr = f + ty;
+        double Fh[4] = {0.0, 0.0, 0.0, 0.0};
+        if (col_ok && fr < F1) {
+            double Ul[4], Ur[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {  // points fr-1 and fr
-                Ul[r] = sU[r][(fr - 1 - base) % kRawRing][cx];
-                Ur[r] = sU[r][(fr - base) % kRawRing][cx];
+                Ul[r] = sU[r][(fr - 1 - base) % kRawRing][tx];
+                Ur[r] = sU[r][(fr - base) % kRawRing][tx];
             }
             const int s0 = fr - 3 - base;  // ring position of point row fr-3
             face_flux_dev<1, S, kCapRecip && S == kZPI>(
-                Ul, Ur, [&](int s, int r, int z) { return sP[r][(s0 + s) % kRing][cx + z]; },
-                [&](int s, int r, int z) { return sM[r][(s0 + s) % kRing][cx + z]; }, g.gamma, g.zp_lambda, Fh);
-            const size_t o = (size_t)(fr - 4) * g.fy_pitch + (ic - 4);
+                Ul, Ur, [&](int s, int r, int z) { return sP[r][(s0 + s) % kRing][tx + z]; },
+                [&](int s, int r, int z) { return sM[r][(s0 + s) % kRing][tx + z]; }, g.gamma, g.zp_lambda, Fh);
+        }
+        const int frc = fr < F1 ? fr : F1 - 1;  // a face row of the array for the resource base
+        const unsigned off = (col_ok && fr < F1) ? lo8 : 0x80000000u;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) Fy[r * g.fy_plane + o] = Fh[r];
-        };
-        const int fr = f + ty;
-        if (col_ok && fr < F1) full_face(tx, fr);
+        for (int r = 0; r < 4; ++r)
+            store_lane(Fy + (size_t)r * g.fy_plane + (size_t)(frc - 4) * g.fy_pitch, g.fy_pitch, off, Fh[r]);
         __syncthreads();
     }
 }
