@@ -326,6 +326,48 @@ def hybrid(W, n, seed, t_end, truth, fp):
     return out
 
 
+def hybrid_breakdown(W, n, seed, t_end, fp, interval, radius):
+    """Per-kernel time per step (HIP events on the solver's stream, summed
+    over the whole run) of plain WENO-Z and of the learned switch with one
+    (interval, radius), on the hybrid comparison's problem, plus the switch's
+    flag-rate trajectory (faces with any field flagged, and fields, every 256
+    steps). The events separate the launches, so the sums run a little above
+    the un-instrumented step time."""
+    P, thr = W.load_switch()
+    keys = dict(TESTCASE=5, SCHEME=1, RKMETH=2, CFL=0.6, ELEMNUMX=n, ELEMNUMY=n, QUAD_SEED=seed,
+                TEND="%.17g" % t_end, FP_CONTRACT=FP_MODES[fp])
+    out = {"workload": f"seeded four-quadrant problem (seed {seed}) {n}x{n}, WENO-Z RK3 CFL 0.6, t_end {t_end}",
+           "interval": interval, "dilation": radius}
+    with W.Solver(**keys) as s:
+        s.set_timing(True)
+        steps = s.advance(0)
+        s.synchronize()
+        kt = s.get_timing()
+        out["plain"] = {"steps": steps, "us_per_step": {k: v[0] / steps * 1e3 for k, v in kt.items() if v[1]}}
+    with W.Solver(**keys) as s:
+        s.switch_set(W.SWITCH_MLP, P, thr)
+        s.switch_interval(interval)
+        s.switch_dilation(radius)
+        s.set_timing(True)
+        total, traj = 0, []
+        while True:
+            k = s.advance(256)
+            total += k
+            fx, fy = s.switch_flags()
+            m = np.concatenate([fx.ravel(), fy.ravel()])
+            traj.append([total, float((m != 0).mean()), float(np.unpackbits(m[:, None], axis=1)[:, 4:].mean())])
+            if k < 256:
+                break
+        s.synchronize()
+        kt = s.get_timing()
+        out["hybrid"] = {"steps": total, "us_per_step": {k: v[0] / total * 1e3 for k, v in kt.items() if v[1]},
+                         "launches": {k: v[1] for k, v in kt.items() if v[1]},
+                         "fallbacks": s.switch_fallbacks(), "flag_trajectory": traj}
+    for side in ("plain", "hybrid"):
+        out[side]["us_per_step_total"] = sum(out[side]["us_per_step"].values())
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -355,6 +397,9 @@ def main():
     ap.add_argument("--hybrid-tend", type=float, default=0.25)
     ap.add_argument("--no-hybrid-truth", dest="hybrid_truth", action="store_false",
                     help="skip the comparison with plain WENO-Z on the nested (2N-1)^2 grid")
+    ap.add_argument("--hybrid-breakdown", default="",
+                    help="INTERVAL:RADIUS: only print the per-kernel time per step of plain WENO-Z and of the "
+                         "learned switch over the hybrid comparison's whole run, with the flag-rate trajectory")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this solver runs one whole domain on one GPU; --gpus must be 1", file=sys.stderr)
@@ -365,6 +410,12 @@ def main():
         return 0
 
     import wenofd as W
+
+    if args.hybrid_breakdown:
+        iv, rad = (int(v) for v in args.hybrid_breakdown.split(":"))
+        print(json.dumps(hybrid_breakdown(W, args.hybrid_size, args.hybrid_seed, args.hybrid_tend, args.fp, iv, rad)),
+              flush=True)
+        return 0
 
     n, sc = args.size, SCHEMES[args.scheme]
     if args.graph_steps != 16:
