@@ -13,7 +13,7 @@
 //   features of characteristic field k at a face (10 floats): the + stencil
 //   v+_k[0..4] and the mirrored - stencil v-_k[4..0], each centred on its mean
 //   and scaled by 1 / (max |deviation| + 1e-10), rounded to float
-//   label: max over both sides and j of |omega_j - d_j| > 0.05 (WENO-Z weights)
+//   label: max over both sides and j of |omega_j - d_j| > 0.2 (WENO-Z weights)
 //   MLP 10 -> 32 -> 32 -> 1 with ReLU, one logit per (face, field);
 //   a face is flagged when any of its 4 fields has a logit above the threshold.
 #pragma once
@@ -34,7 +34,7 @@ constexpr int kOffB2 = kOffW2 + kHid * kHid;   // 1376
 constexpr int kOffW3 = kOffB2 + kHid;          // 1408
 constexpr int kOffB3 = kOffW3 + kHid;          // 1440
 constexpr int kNParams = kOffB3 + 1;           // 1441
-constexpr double kLabelThreshold = 0.05;
+constexpr double kLabelThreshold = 0.2;
 constexpr double kFeatScaleEps = 1e-10;
 
 template <class T>

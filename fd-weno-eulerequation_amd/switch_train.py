@@ -4,7 +4,7 @@
    u, v ~ U[-1.5, 1.5] per quadrant) at N^2, WENO-Z RK3 CFL 0.6, run to
    t_end; at each snapshot time every face of both directions gives 4 samples
    (one per characteristic field: 10 normalised stencil values, label
-   max |omega - d| > 0.05), kept with probability keep_pos / keep_neg.
+   max |omega - d| > 0.2), kept with probability keep_pos / keep_neg.
    Training seeds 0 .. seeds-val-1, validation seeds after them.
 2. Train: the library's FP32 forward/backward/Adam kernels, weighted BCE.
 3. Threshold: the logit at which the deployed bf16 MFMA network keeps the

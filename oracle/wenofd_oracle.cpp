@@ -427,6 +427,10 @@ static inline void face_char(const double* Ul, const double* Ur, const double (*
         }
 }
 
+// Label of a stencil pair: WENO-Z's weights leave the linear ones by more
+// than this on either side.
+static const double kLabelThreshold = 0.2;
+
 // Features x[k][0..9] (+ stencil, then the mirrored - stencil) and labels of a face.
 static inline void face_samples(const double vp[5][4], const double vm[5][4], float* x40, unsigned char* lab4) {
     for (int k = 0; k < 4; ++k) {
@@ -436,7 +440,7 @@ static inline void face_samples(const double vp[5][4], const double vm[5][4], fl
         stencil_features(wm, x40 + 10 * k + 5);
         const double dp = weno_z_departure(wp[0], wp[1], wp[2], wp[3], wp[4]);
         const double dm = weno_z_departure(wm[0], wm[1], wm[2], wm[3], wm[4]);
-        lab4[k] = (dp > 0.05 || dm > 0.05) ? 1 : 0;
+        lab4[k] = (dp > kLabelThreshold || dm > kLabelThreshold) ? 1 : 0;
     }
 }
 
