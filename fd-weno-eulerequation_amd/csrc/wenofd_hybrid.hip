@@ -336,8 +336,11 @@ __global__ __launch_bounds__(kHybCols, kHybWaves) void k_hyb_stage(const double*
     auto lane_u = [&](const unsigned* row_ptr, unsigned off) -> unsigned {
         return *reinterpret_cast<const unsigned*>(reinterpret_cast<const char*>(row_ptr) + off);
     };
-    auto load_u = [&](int row, double u[4]) {  // U of row (clamped)
-        row = row < 0 ? 0 : (row < rmax ? row : rmax);
+    // U of row, clamped to the rows the march splits (the prefetches past its
+    // last row re-read that row instead of fetching rows it never uses)
+    const int qtop = qe < rmax ? qe : rmax;
+    auto load_u = [&](int row, double u[4]) {
+        row = row < 0 ? 0 : (row < qtop ? row : qtop);
         const double* p = Uin + (size_t)row * g.pitch;
 #pragma unroll
         for (int r = 0; r < 4; ++r) u[r] = lane_d(p + (size_t)r * g.plane);
