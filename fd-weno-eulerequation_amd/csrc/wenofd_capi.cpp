@@ -96,14 +96,12 @@ struct wenofd_solver {
     unsigned char* fl_x = nullptr;
     unsigned char* fl_y = nullptr;
     // The faces of the dilated masks that flag any field (rebuilt whenever the
-    // masks change): the list (grouped by flagged-field count), each face's
-    // slot (its list position, ~0 for none), the faces' summed corrections (4
-    // doubles each, per stage), the face counts on the device and the list
+    // masks change): the list (grouped by flagged-field count), the summed
+    // corrections of the listed faces at each face's index (4 doubles per
+    // face, written per stage), the face counts on the device and the list
     // build's scratch.
     unsigned* ent_x = nullptr;
     unsigned* ent_y = nullptr;
-    unsigned* slot_x = nullptr;
-    unsigned* slot_y = nullptr;
     double* corr_x = nullptr;
     double* corr_y = nullptr;
     int* n_pairs = nullptr;  // {x, y}
@@ -209,7 +207,7 @@ int stage(wenofd_solver* h, double* in, const double* un, double* out, const RkC
         if ((rc = timed(h, WENOFD_K_HYB, [&] {
                  return WENOFD_FP(fp, launch_hyb_stage(in, un, out, rhs_out, h->g, k, h->opt.cfl,
                                                        last_stage ? h->dt_part : nullptr,
-                                                       last_stage ? h->dt_part_nan : nullptr, h->slot_x, h->slot_y,
+                                                       last_stage ? h->dt_part_nan : nullptr, h->fl_x, h->fl_y,
                                                        h->corr_x, h->corr_y, segs, h->stream));
              })))
             return rc;
@@ -236,12 +234,12 @@ int stage(wenofd_solver* h, double* in, const double* un, double* out, const RkC
 // in both directions of the face grid: the masks are computed from the
 // initial state of every sw_every-th step and serve all stages of that many
 // steps (a wave moves less than one cell per step at CFL <= 1).
-// The pair lists and slots of the current dilated masks fl_x / fl_y.
+// The face lists of the current dilated masks fl_x / fl_y.
 hipError_t build_lists(wenofd_solver* h) {
-    hipError_t e = launch_build_list(h->fl_x, h->faces_x(), h->list_cnt, h->list_off, h->ent_x, h->slot_x,
+    hipError_t e = launch_build_list(h->fl_x, h->faces_x(), h->list_cnt, h->list_off, h->ent_x,
                                      h->n_pairs, h->stream);
     if (e == hipSuccess)
-        e = launch_build_list(h->fl_y, h->faces_y(), h->list_cnt, h->list_off, h->ent_y, h->slot_y, h->n_pairs + 1,
+        e = launch_build_list(h->fl_y, h->faces_y(), h->list_cnt, h->list_off, h->ent_y, h->n_pairs + 1,
                               h->stream);
     return e;
 }
@@ -764,7 +762,7 @@ int wenofd_destroy(wenofd_solver* h) {
     (void)hipFree(h->fl_raw_y);
     (void)hipFree(h->fl_x);
     (void)hipFree(h->fl_y);
-    for (void* p : {(void*)h->ent_x, (void*)h->ent_y, (void*)h->slot_x, (void*)h->slot_y, (void*)h->corr_x,
+    for (void* p : {(void*)h->ent_x, (void*)h->ent_y, (void*)h->corr_x,
                     (void*)h->corr_y, (void*)h->n_pairs, (void*)h->list_cnt, (void*)h->list_off})
         (void)hipFree(p);
     if (h->dt_host) (void)hipHostFree(h->dt_host);
@@ -1126,11 +1124,9 @@ int wenofd_switch_set(wenofd_solver* h, int mode, const float* params, float thr
         HIP_TRY(hipMalloc(&h->fl_raw_y, fy));
         HIP_TRY(hipMalloc(&h->fl_x, fx));
         HIP_TRY(hipMalloc(&h->fl_y, fy));
-        // up to every face flagged
+        // up to every face flagged; corrections at the face's index
         HIP_TRY(hipMalloc(&h->ent_x, (size_t)fx * sizeof(unsigned)));
         HIP_TRY(hipMalloc(&h->ent_y, (size_t)fy * sizeof(unsigned)));
-        HIP_TRY(hipMalloc(&h->slot_x, (size_t)fx * sizeof(unsigned)));
-        HIP_TRY(hipMalloc(&h->slot_y, (size_t)fy * sizeof(unsigned)));
         HIP_TRY(hipMalloc(&h->corr_x, 4 * (size_t)fx * sizeof(double)));
         HIP_TRY(hipMalloc(&h->corr_y, 4 * (size_t)fy * sizeof(double)));
         HIP_TRY(hipMalloc(&h->n_pairs, 2 * sizeof(int)));

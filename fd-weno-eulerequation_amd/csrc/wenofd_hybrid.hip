@@ -24,8 +24,9 @@
 //   x faces: from a wave-local LDS ring of the last four rows, each lane
 //            splits the point two columns to its right and the face sums
 //            travel between lanes by DPP shifts;
-//   a face with flagged fields adds their summed correction (its slot: its
-//   position in the list), loaded a row ahead, its slot further ahead.
+//   a face with flagged fields adds their summed correction (stored at the
+//   face's index), loaded a row ahead once the face's mask, loaded further
+//   ahead, says it has one.
 // The grid is one round of resident waves (no tail), strips and segments
 // ordered so that each XCD takes neighbouring ones.
 // Compiled twice, like wenofd_kernels.hip: exact (no contraction) and fused.
@@ -216,11 +217,11 @@ __device__ __forceinline__ void patch_run(const double* __restrict__ Uin, const 
     for (int e = e0; e < n; e += 2 * stride) {
         const int eb = e + stride;
         unsigned fb = entry(eb);
-        patch_face<S, D>(Uin, g, fa, ma & 15u, corr + (size_t)e * 4);
+        patch_face<S, D>(Uin, g, fa, ma & 15u, corr + (size_t)fa * 4);
         unsigned mb = masks[fb];
         if (eb >= n) break;
         fa = entry(eb + stride);
-        patch_face<S, D>(Uin, g, fb, mb & 15u, corr + (size_t)eb * 4);
+        patch_face<S, D>(Uin, g, fb, mb & 15u, corr + (size_t)fb * 4);
         ma = masks[fa];
     }
 }
@@ -245,12 +246,11 @@ __global__ __launch_bounds__(kPatchThreads) void k_hyb_patch(const double* __res
     else patch_run<S, 1>(Uin, g, list_y, my, n, corr_y, e0, stride);
 }
 
-// The summed correction of a face's flagged fields (loaded when its slot is
-// not ~0).
-__device__ __forceinline__ void load_correction(unsigned slot, const double* __restrict__ corr, double c[4]) {
-    if (slot != ~0u) {
+// The summed correction of face `face`'s flagged fields (loaded when it has any).
+__device__ __forceinline__ void load_correction(bool on, size_t face, const double* __restrict__ corr, double c[4]) {
+    if (on) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) c[r] = corr[(size_t)slot * 4 + r];
+        for (int r = 0; r < 4; ++r) c[r] = corr[face * 4 + r];
     }
 }
 
@@ -261,7 +261,7 @@ __device__ __forceinline__ void load_correction(unsigned slot, const double* __r
 // register with a load in flight waits for that load). The corrections of the
 // faces of row q + 1 are loaded first thing at row q (vector memory counts
 // complete in issue order, so nothing issued earlier is waited for early),
-// from slots loaded kHybAhead rows before that. Every load reads a valid
+// from masks loaded kHybAhead rows before that. Every load reads a valid
 // address (columns and rows clamped into the array): a load under a lane
 // condition with the other lanes' registers set otherwise would also wait for
 // the load.
@@ -270,8 +270,9 @@ constexpr int kHybAhead = 2;
 // One RK stage of the hybrid solver on one strip of columns and segment of rows:
 //   kRkNone:  rhs only (rhs_out)     kRk1: out = in + dt rhs
 //   kRkStage: out = (a Un + b in) + (c dt) rhs
-// slot_x / slot_y: per face (x faces [ny][nx+1], y faces [ny+1][nx]) the
-// slot of its flagged fields' corrections in corr_x / corr_y. With part set,
+// mask_x / mask_y: per face (x faces [ny][nx+1], y faces [ny+1][nx]) the
+// flagged fields (bits 0-3), whose summed correction is at the face's index in
+// corr_x / corr_y. With part set,
 // the block's CFL partial minimum of its updated cells.
 // step(j, q0 + j) for j = 0 .. NB-1, j a compile-time constant.
 template <class F, int... J>
@@ -295,8 +296,8 @@ __global__ __launch_bounds__(kHybCols, kHybWaves) void k_hyb_stage(const double*
                                                                    double* Uout, double* __restrict__ rhs_out, Geom g,
                                                                    RkCoef k, double cfl, double* __restrict__ part,
                                                                    int* __restrict__ part_nan,
-                                                                   const unsigned* __restrict__ slot_x,
-                                                                   const unsigned* __restrict__ slot_y,
+                                                                   const unsigned char* __restrict__ mask_x,
+                                                                   const unsigned char* __restrict__ mask_y,
                                                                    const double* __restrict__ corr_x,
                                                                    const double* __restrict__ corr_y,
                                                                    const SolveState* __restrict__ st) {
@@ -324,17 +325,14 @@ __global__ __launch_bounds__(kHybCols, kHybWaves) void k_hyb_stage(const double*
     const int qa = r0 - 3, qe = r1 + 2;  // the rows split
     const int rmax = g.ny + 7;
     const bool gh_x = k.gh.on && ghost_x_edge(g, c0, c0 + kHybOut - 1);
-    // byte offsets of this lane's point in a row (clamped into the row) and
-    // of its y face / x face in a face row
+    // byte offset of this lane's point in a row (clamped into the row), and
+    // the index of its y face / x face in a face row (clamped)
     const unsigned lo8 = (unsigned)(i < 0 ? 0 : (i < g.nx + 7 ? i : g.nx + 7)) * 8u;
-    const unsigned ly4 = (unsigned)((i < 4 ? 4 : (i > g.nx + 3 ? g.nx + 3 : i)) - 4) * 4u;
-    const unsigned lx4 = (unsigned)((i < 4 ? 4 : (i > g.nx + 4 ? g.nx + 4 : i)) - 4) * 4u;
+    const unsigned ly1 = (unsigned)((i < 4 ? 4 : (i > g.nx + 3 ? g.nx + 3 : i)) - 4);
+    const unsigned lx1 = (unsigned)((i < 4 ? 4 : (i > g.nx + 4 ? g.nx + 4 : i)) - 4);
 
     auto lane_d = [&](const double* row_ptr) -> double {
         return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(row_ptr) + lo8);
-    };
-    auto lane_u = [&](const unsigned* row_ptr, unsigned off) -> unsigned {
-        return *reinterpret_cast<const unsigned*>(reinterpret_cast<const char*>(row_ptr) + off);
     };
     // U of row, clamped to the rows the march splits (the prefetches past its
     // last row re-read that row instead of fetching rows it never uses)
@@ -366,18 +364,18 @@ __global__ __launch_bounds__(kHybCols, kHybWaves) void k_hyb_stage(const double*
             __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v[r]), rs, off, 0, 0);
         }
     };
-    // raw slots of the faces of row q: y face q-2 and the x faces of cell row q-3 (clamped)
-    auto load_slots = [&](int q, unsigned& sy, unsigned& sx) {
+    // masks of the faces of row q: y face q-2 and the x faces of cell row q-3 (clamped)
+    auto load_masks = [&](int q, unsigned& sy, unsigned& sx) {
         int fr = q - 2, cr = q - 3;
         fr = fr < 4 ? 4 : (fr > g.ny + 4 ? g.ny + 4 : fr);
         cr = cr < 4 ? 4 : (cr > g.ny + 3 ? g.ny + 3 : cr);
-        sy = lane_u(slot_y + (size_t)(fr - 4) * g.nx, ly4);
-        sx = lane_u(slot_x + (size_t)(cr - 4) * (g.nx + 1), lx4);
+        sy = mask_y[(size_t)(fr - 4) * g.nx + ly1];
+        sx = mask_x[(size_t)(cr - 4) * (g.nx + 1) + lx1];
     };
 
     double ub[NB][4];     // U of the rows of the group, loaded NB rows ahead
     double unb[NB][4];    // Un of their cell rows
-    unsigned syb[NB], sxb[NB];  // raw slots of the faces of rows q + 1 (loaded NB rows before row q)
+    unsigned syb[NB], sxb[NB];  // masks of the faces of rows q + 1 (loaded NB rows before row q)
     double cyb[2][4], cxb[2][4];  // corrections of the faces of rows q (set q & 1), where flagged
     bool fyb_on[2] = {false, false}, fxb_on[2] = {false, false};  // which lanes have one
     double acc[5][4];   // acc[m]: partial cheap flux of face q-2+m (before row q)
@@ -399,7 +397,7 @@ __global__ __launch_bounds__(kHybCols, kHybWaves) void k_hyb_stage(const double*
 #pragma unroll
         for (int r = 0; r < 4; ++r) ub[j][r] = unb[j][r] = 0.0;
         ub[j][0] = ub[j][3] = 1.0;  // a finite state for the early rows' splits
-        syb[j] = sxb[j] = ~0u;
+        syb[j] = sxb[j] = 0u;
     }
 
     auto step = [&](auto J, int q) {
@@ -409,14 +407,12 @@ __global__ __launch_bounds__(kHybCols, kHybWaves) void k_hyb_stage(const double*
         // the corrections of the faces of row q + 1 this block owns
         {
             const int fr = q + 1 - 2, cr = q + 1 - 3;
-            const unsigned sy = (out_col && fr >= r0) ? syb[j] : ~0u;
-            const unsigned sx = (x_face && cr >= r0) ? sxb[j] : ~0u;
-            fyb_on[cn] = sy != ~0u;
-            fxb_on[cn] = sx != ~0u;
-            load_correction(sy, corr_y, cyb[cn]);
-            load_correction(sx, corr_x, cxb[cn]);
+            fyb_on[cn] = out_col && fr >= r0 && (syb[j] & 15u) != 0u;
+            fxb_on[cn] = x_face && cr >= r0 && (sxb[j] & 15u) != 0u;
+            load_correction(fyb_on[cn], (size_t)(fr - 4) * g.nx + ly1, corr_y, cyb[cn]);
+            load_correction(fxb_on[cn], (size_t)(cr - 4) * (g.nx + 1) + lx1, corr_x, cxb[cn]);
             reload_fence();
-            load_slots(q + 1 + NB, syb[j], sxb[j]);
+            load_masks(q + 1 + NB, syb[j], sxb[j]);
         }
         // ---- row q: y split, partial sums of faces q-2 .. q+3 ----
         double fm[4], fp[4], fyt[4];
@@ -564,13 +560,14 @@ hipError_t launch_hyb_patch(int scheme, const double* Uin, const Geom& g, const 
 }
 
 hipError_t launch_hyb_stage(const double* Uin, const double* Un, double* Uout, double* rhs_out, const Geom& g,
-                            const RkCoef& k, double cfl, double* dt_part, int* dt_nan, const unsigned* slot_x,
-                            const unsigned* slot_y, const double* corr_x, const double* corr_y, int segs,
+                            const RkCoef& k, double cfl, double* dt_part, int* dt_nan,
+                            const unsigned char* mask_x, const unsigned char* mask_y, const double* corr_x,
+                            const double* corr_y, int segs,
                             hipStream_t s, const SolveState* st) {
     const dim3 grid(hyb_strips(g), segs);
     auto go = [&](auto M) {
         hipLaunchKernelGGL((k_hyb_stage<kHybAhead, decltype(M)::value>), grid, dim3(kHybCols), 0, s, Uin, Un, Uout,
-                           rhs_out, g, k, cfl, dt_part, dt_nan, slot_x, slot_y, corr_x, corr_y, st);
+                           rhs_out, g, k, cfl, dt_part, dt_nan, mask_x, mask_y, corr_x, corr_y, st);
     };
     if (k.mode == kRkNone) go(std::integral_constant<int, kRkNone>());
     else if (k.mode == kRk1) go(std::integral_constant<int, kRk1>());

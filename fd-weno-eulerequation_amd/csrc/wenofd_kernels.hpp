@@ -158,11 +158,11 @@ hipError_t launch_dilate(const unsigned char* in, unsigned char* out, int rows, 
 hipError_t launch_fill_u8(unsigned char* p, long n, unsigned char v, hipStream_t s);
 // The faces of one direction's masks (F faces) that flag any field: list[]
 // grouped by the number of flagged fields (1 .. 4), each group in face order;
-// slot[face] = its position in list, or ~0; *n = the number of faces (device).
-// cnt / off: 4 list_blocks(F) ints of scratch each.
+// *n = the number of faces (device). cnt / off: 4 list_blocks(F) ints of
+// scratch each.
 long list_blocks(long F);
-hipError_t launch_build_list(const unsigned char* masks, long F, int* cnt, int* off, unsigned* list, unsigned* slot,
-                             int* n, hipStream_t s);
+hipError_t launch_build_list(const unsigned char* masks, long F, int* cnt, int* off, unsigned* list, int* n,
+                             hipStream_t s);
 hipError_t launch_mlp_infer(const float* P, const float* X, long n, float* z, hipStream_t s);
 hipError_t launch_keep_count(const unsigned char* Y, long n, unsigned long long seed, double p_pos, double p_neg,
                              int* counts, hipStream_t s);

@@ -280,9 +280,8 @@ __global__ __launch_bounds__(256) void k_dilate(const unsigned char* __restrict_
 // number of flagged fields (1 .. 4; within a group in face order), so that a
 // wave of the correction kernel runs one field count: k_list_count counts
 // each group in each run of kListFaces faces, k_list_scan turns the counts
-// into offsets (one block), k_list_write writes the faces and each face's
-// slot (its list position, ~0 for an unflagged face). Deterministic: every
-// position depends on the masks alone.
+// into offsets (one block), k_list_write writes the faces. Deterministic:
+// every position depends on the masks alone.
 // ---------------------------------------------------------------------------
 constexpr int kListFaces = 1024;  // faces per counting block (4 per thread)
 
@@ -344,8 +343,7 @@ __global__ __launch_bounds__(1024) void k_list_scan(const int* __restrict__ cnt,
 }
 
 __global__ __launch_bounds__(256) void k_list_write(const unsigned char* __restrict__ m, long F,
-                                                    const int* __restrict__ off, unsigned* __restrict__ list,
-                                                    unsigned* __restrict__ slot) {
+                                                    const int* __restrict__ off, unsigned* __restrict__ list) {
     __shared__ unsigned long long s[256];
     const int t = threadIdx.x;
     const long f0 = (long)blockIdx.x * kListFaces + 4 * t;
@@ -368,14 +366,10 @@ __global__ __launch_bounds__(256) void k_list_write(const unsigned char* __restr
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (f0 + q >= F) break;
-        if (!mk[q]) {
-            slot[f0 + q] = ~0u;
-            continue;
-        }
+        if (!mk[q]) continue;
         const int grp = __popc(mk[q]) - 1;
         const unsigned e = (unsigned)off[4 * blockIdx.x + grp] + (unsigned)((pos >> (16 * grp)) & 0xFFFFull);
         pos += 1ull << (16 * grp);
-        slot[f0 + q] = e;
         list[e] = (unsigned)(f0 + q);
     }
 }
@@ -617,12 +611,12 @@ hipError_t launch_dilate(const unsigned char* in, unsigned char* out, int rows, 
 
 long list_blocks(long F) { return (F + kListFaces - 1) / kListFaces; }
 
-hipError_t launch_build_list(const unsigned char* masks, long F, int* cnt, int* off, unsigned* list, unsigned* slot,
-                             int* n, hipStream_t s) {
+hipError_t launch_build_list(const unsigned char* masks, long F, int* cnt, int* off, unsigned* list, int* n,
+                             hipStream_t s) {
     const long nb = list_blocks(F);
     hipLaunchKernelGGL(k_list_count, dim3(nb), dim3(256), 0, s, masks, F, cnt);
     hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(1024), 0, s, cnt, (int)nb, off, n);
-    hipLaunchKernelGGL(k_list_write, dim3(nb), dim3(256), 0, s, masks, F, off, list, slot);
+    hipLaunchKernelGGL(k_list_write, dim3(nb), dim3(256), 0, s, masks, F, off, list);
     return hipGetLastError();
 }
 
