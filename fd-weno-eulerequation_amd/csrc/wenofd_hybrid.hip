@@ -55,25 +55,6 @@ constexpr int kHybMinRows = 8;           // fewest output rows per segment
 
 __host__ __device__ inline int hyb_strips(const Geom& g) { return (g.nx + kHybOut - 1) / kHybOut; }
 
-// The cell terms with the spacings (as the plain x sweep: the build that
-// divides by the reciprocal multiplies by the host's 1/dx and 1/dy).
-__device__ __forceinline__ double hyb_cell_rhs(double fxl, double fxr, double fyb, double fyt, const Geom& g) {
-#if WENOFD_RECIP_DIV
-    return -(fxr - fxl) * g.rdx + -(fyt - fyb) * g.rdy;
-#else
-    return cell_rhs(fxl, fxr, fyb, fyt, g.dx, g.dy);
-#endif
-}
-
-__device__ __forceinline__ double hyb_local_dt(const double U[4], const Geom& g, double cfl) {
-#if WENOFD_RECIP_DIV
-    const Prim<double> q = primitive(U[0], U[1], U[2], U[3], g.gamma);
-    return cfl / ((fabs(q.u) + q.c) * g.rdx + (fabs(q.v) + q.c) * g.rdy);
-#else
-    return local_dt(U, g.gamma, cfl, g.dx, g.dy);
-#endif
-}
-
 __device__ __forceinline__ double hyb_wave_min(double v) {
     // every lane active; MIN is exact, so the exchange order does not matter
 #pragma unroll
@@ -468,8 +449,7 @@ __global__ __launch_bounds__(kHybCols, kHybWaves) void k_hyb_stage(const double*
                 }
                 if (out_col) {
                     double rhs[4];
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) rhs[r] = hyb_cell_rhs(fl[r], fr[r], fyb[r], fyt[r], g);
+                    cell_rhs4(fl, fr, fyb, fyt, g.dx, g.dy, g.rdx, g.rdy, rhs);
                     if (k.gravity) {  // src/WENOFD.cpp:812-813
                         rhs[3] = rhs[3] + uc[2];
                         rhs[2] = rhs[2] + uc[0];
@@ -499,7 +479,7 @@ __global__ __launch_bounds__(kHybCols, kHybWaves) void k_hyb_stage(const double*
                                 });
                         }
                         if (part) {
-                            const double tl = hyb_local_dt(w, g, cfl);
+                            const double tl = local_dt_dev(w, g.gamma, cfl, g.dx, g.dy, g.rdx, g.rdy);
                             if (tl != tl) my_nan = 1;
                             my_min = (tl < my_min) ? tl : my_min;
                         }

@@ -218,14 +218,14 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
     }
 }
 
-// The WENO-ZP y sweep wants a few more registers than 3 waves per SIMD
-// allow; capping it there (a 3-dword spill) runs 2 % faster than 2 waves
-// (for WENO-ZPI the 5-dword spill costs more than it gains). The build that
-// divides by the reciprocal caps every scheme: JS and Z would otherwise take
-// 246 VGPRs, and its ZPI sweep runs 4 % faster capped (a 9-dword spill) than
-// at 2 waves.
+// The WENO-Z and WENO-ZP y sweeps want a few more registers than 3 waves
+// per SIMD allow (Z: 172 VGPRs); capped there (a 1- and a 3-dword spill)
+// they run faster than at 2 waves (ZP: 2 %); for WENO-ZPI the 5-dword
+// spill costs more than it gains. The build that divides by the reciprocal
+// caps every scheme: JS and Z would otherwise take 246 VGPRs, and its ZPI
+// sweep runs 4 % faster capped (a 9-dword spill) than at 2 waves.
 template <int S>
-constexpr int y_min_waves() { return S == kZP || kCapRecip ? 3 : 1; }
+constexpr int y_min_waves() { return S == kZ || S == kZP || kCapRecip ? 3 : 1; }
 
 template <int S, int YF>
 __global__ __launch_bounds__(kYC* kYR, (y_min_waves<S>())) void k_flux_y(const double* __restrict__ U,
@@ -262,24 +262,10 @@ __device__ __forceinline__ double wave_min(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// The cell terms with the spacings. The build that divides by the reciprocal
-// multiplies by the host's 1/dx and 1/dy, the reciprocals its own divisions
-// would otherwise compute in every thread.
-__device__ __forceinline__ double cell_rhs_g(double fxl, double fxr, double fyb, double fyt, const Geom& g) {
-#if WENOFD_RECIP_DIV
-    return -(fxr - fxl) * g.rdx + -(fyt - fyb) * g.rdy;
-#else
-    return cell_rhs(fxl, fxr, fyb, fyt, g.dx, g.dy);
-#endif
-}
-
+// The cell terms with the spacings (cell_rhs4, local_dt_dev: the build that
+// divides by the reciprocal multiplies by the host's 1/dx and 1/dy).
 __device__ __forceinline__ double local_dt_g(const double U[4], const Geom& g, double cfl) {
-#if WENOFD_RECIP_DIV
-    const Prim<double> q = primitive(U[0], U[1], U[2], U[3], g.gamma);
-    return cfl / ((fabs(q.u) + q.c) * g.rdx + (fabs(q.v) + q.c) * g.rdy);
-#else
-    return local_dt(U, g.gamma, cfl, g.dx, g.dy);
-#endif
+    return local_dt_dev(U, g.gamma, cfl, g.dx, g.dy, g.rdx, g.rdy);
 }
 
 // ---------------------------------------------------------------------------
@@ -378,12 +364,14 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
     double my_min = 1.0;
     int my_nan = 0;
     if (cell_ok) {
-        double rhs[4], u[4];
+        double rhs[4], u[4], fxl[4], fxr[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            rhs[r] = cell_rhs_g(sF[r][t], sF[r][t + 1], fyb[r], fyt[r], g);
+            fxl[r] = sF[r][t];
+            fxr[r] = sF[r][t + 1];
             u[r] = sU[r][t + 3];
         }
+        cell_rhs4(fxl, fxr, fyb, fyt, g.dx, g.dy, g.rdx, g.rdy, rhs);
         if (k.gravity) {  // src/WENOFD.cpp:812-813
             rhs[3] = rhs[3] + u[2];
             rhs[2] = rhs[2] + u[0];

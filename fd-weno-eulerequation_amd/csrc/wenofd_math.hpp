@@ -153,6 +153,32 @@ __device__ __forceinline__ double div_shared(double n, double d, double r) {
 #endif
 }
 
+// The build without contraction divides by the density, the sound speed,
+// its square and the spacings through shared refined reciprocals as well
+// (WENOFD_SHARED_DIV): div_exact is bit-identical to the IEEE division when
+// the denominator d lies in [2^-300, 2^300] (checked once per denominator,
+// den_ok) and the numerator is +-0 or its magnitude lies in [2^-600, 2^301)
+// (its biased exponent in [423, 1323]): no div_scale then rescales, and
+// div_fixup returns the refined quotient, or for a zero numerator the zero of
+// the quotient's sign, which n r already is (the two FMAs would turn -0 into
+// +0). ok is cleared otherwise (NaN and infinity included), and the caller
+// redoes the lane's divisions in IEEE arithmetic.
+#if !WENOFD_RECIP_DIV && defined(__HIP_DEVICE_COMPILE__)
+#define WENOFD_SHARED_DIV 1
+#else
+#define WENOFD_SHARED_DIV 0
+#endif
+
+__device__ __forceinline__ bool den_ok(double d) { return d >= 0x1p-300 && d <= kDivSafeMax; }
+
+__device__ __forceinline__ double div_exact(double n, double d, double r, bool& ok) {
+    const double q0 = n * r;
+    const double q = __builtin_fma(__builtin_fma(-d, q0, n), r, q0);
+    const unsigned e = ((unsigned)__double2hiint(n) >> 20) & 0x7FFu;
+    ok = ok && (e - 423u <= 900u || n == 0.0);
+    return n == 0.0 ? q0 : q;
+}
+
 // weno5 with the weight normalisation (three divisions by the common sum s)
 // and WENO-Z's reciprocals 1/(beta + 1e-20) done by div_refined. ok is cleared
 // when an operand leaves the exact range: s >= 1 and 0.1 <= w_k <= s always
@@ -291,7 +317,8 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
 
 // Primitive variables with the reference's expression forms
 // (utils/equations.hpp:139-144): p = (g-1)(E - (0.5 rho)(u^2+v^2)), c = sqrt((g p)/rho).
-// irho = 1/rho is set only where WENOFD_RCP_STATE computes it.
+// irho = 1/rho is set only where WENOFD_RCP_STATE computes it, or (the
+// refined reciprocal) where WENOFD_SHARED_DIV does.
 template <class T>
 struct Prim {
     T rho, u, v, E, p, c, irho;
@@ -322,12 +349,45 @@ WENOFD_HD Prim<T> primitive(T U0, T U1, T U2, T U3, double gamma) {
     return q;
 }
 
+#if WENOFD_SHARED_DIV
+// primitive through the density's refined reciprocal (div_exact; irho = the
+// reciprocal); ok is cleared where a division leaves the exact range, and the
+// caller then takes primitive's IEEE result for that lane.
+__device__ __forceinline__ Prim<double> primitive_sd(double U0, double U1, double U2, double U3, double gamma,
+                                                     bool& ok) {
+    Prim<double> q;
+    ok = ok && den_ok(U0);
+    const double r = rcp_refined(U0);
+    q.rho = U0;
+    q.irho = r;
+    q.u = div_exact(U1, U0, r, ok);
+    q.v = div_exact(U2, U0, r, ok);
+    q.E = U3;
+    q.p = (gamma - 1.0) * (q.E - (0.5 * q.rho) * (q.u * q.u + q.v * q.v));
+    q.c = sqrt(div_exact(gamma * q.p, U0, r, ok));
+    return q;
+}
+#endif
+
 // Local Lax-Friedrichs split at one point along direction D (0 = x, 1 = y):
 // F-+ = 0.5 (F -+ alpha U), alpha = |u_n| + c at this point.
 // utils/equations.hpp:105-134 (physical flux), :365-393 (split).
 template <int D, class T>
 WENOFD_HD void flux_split(const T U[4], double gamma, T Fm[4], T Fp[4]) {
+#if WENOFD_SHARED_DIV
+    Prim<T> q;
+    if constexpr (std::is_same<T, double>::value) {
+        bool ok = true;
+        q = primitive_sd(U[0], U[1], U[2], U[3], gamma, ok);
+        if (__any(!ok)) {
+            if (!ok) q = primitive(U[0], U[1], U[2], U[3], gamma);
+        }
+    } else {
+        q = primitive(U[0], U[1], U[2], U[3], gamma);
+    }
+#else
     const Prim<T> q = primitive(U[0], U[1], U[2], U[3], gamma);
+#endif
     T F[4];
     if constexpr (D == 0) {
         const T ru = q.rho * q.u;
@@ -387,8 +447,7 @@ WENOFD_HD EigenParts<T> eigen_parts(const T U[4], double gamma) {
 // state U, with the reference's normalisation (utils/equations.hpp:174-299).
 // Row-major 4x4: L[i*4+j].
 template <int D, class T>
-WENOFD_HD void eigen_matrices(const T U[4], double gamma, T L[16], T R[16]) {
-    const EigenParts<T> e = eigen_parts<D>(U, gamma);
+WENOFD_HD void eigen_matrices_from(const EigenParts<T>& e, T L[16], T R[16]) {
     const T B1 = e.B1, B2 = e.B2, B1u = e.B1u, B1v = e.B1v, ic = e.ic, H = e.H, uu = e.uu;
     if constexpr (D == 0) {
         const T uc = e.nc;
@@ -411,6 +470,11 @@ WENOFD_HD void eigen_matrices(const T U[4], double gamma, T L[16], T R[16]) {
         R[8] = e.v - e.c;        R[9] = 0.0;   R[10] = e.v;       R[11] = e.v + e.c;
         R[12] = H - e.c * e.v;   R[13] = e.u;  R[14] = uu / 2.0;  R[15] = H + e.c * e.v;
     }
+}
+
+template <int D, class T>
+WENOFD_HD void eigen_matrices(const T U[4], double gamma, T L[16], T R[16]) {
+    eigen_matrices_from<D>(eigen_parts<D>(U, gamma), L, R);
 }
 
 // Row k of L and column k of R of eigen_matrices<D> from its parts (the same
@@ -535,7 +599,10 @@ __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double U
     double ave[4], L[16], R[16];
 #pragma unroll
     for (int r = 0; r < 4; ++r) ave[r] = 0.5 * (Ul[r] + Ur[r]);
+    // (the mean state's divisions stay IEEE: through refined reciprocals they
+    // would cost the x sweep its third wave per SIMD)
     eigen_matrices<D>(ave, gamma, L, R);
+    bool ok = true;  // cleared where a reconstruction's division leaves its exact range
     double vp[5][4], vm[5][4];
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
@@ -548,7 +615,6 @@ __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double U
         project_left<D>(L, fp, vp[s]);
         project_left<D>(L, fm, vm[s]);
     }
-    bool ok = true;
     double vh[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -598,6 +664,73 @@ WENOFD_HD T cell_rhs(T fxl, T fxr, T fyb, T fyt, double dx, double dy) {
     const T tx = -(fxr - fxl) / dx;
     const T ty = -(fyt - fyb) / dy;
     return tx + ty;
+}
+
+// The device forms of local_dt and of cell_rhs for the four conserved
+// variables, given the spacings and their reciprocals rdx = 1/dx, rdy = 1/dy
+// from the host. The build that divides by the reciprocal multiplies by
+// them; the build without contraction divides through refined reciprocals
+// (div_exact: the same quotients), and takes the IEEE divisions in a lane
+// whose operands leave that range.
+__device__ __forceinline__ double local_dt_dev(const double U[4], double gamma, double cfl, double dx, double dy,
+                                               double rdx, double rdy) {
+#if WENOFD_SHARED_DIV
+    bool ok = den_ok(dx) && den_ok(dy);
+    const Prim<double> q = primitive_sd(U[0], U[1], U[2], U[3], gamma, ok);
+#else
+    const Prim<double> q = primitive(U[0], U[1], U[2], U[3], gamma);
+#endif
+    const double ex = fabs(q.u) + q.c;
+    const double ey = fabs(q.v) + q.c;
+#if WENOFD_RECIP_DIV
+    (void)dx;
+    (void)dy;
+    return cfl / (ex * rdx + ey * rdy);
+#elif WENOFD_SHARED_DIV
+    (void)rdx;
+    (void)rdy;
+    const double s = div_exact(ex, dx, rcp_refined(dx), ok) + div_exact(ey, dy, rcp_refined(dy), ok);
+    ok = ok && den_ok(s);
+    double t = div_exact(cfl, s, rcp_refined(s), ok);
+    if (__any(!ok)) {
+        if (!ok) t = local_dt(U, gamma, cfl, dx, dy);
+    }
+    return t;
+#else
+    (void)rdx;
+    (void)rdy;
+    return cfl / (ex / dx + ey / dy);
+#endif
+}
+
+__device__ __forceinline__ void cell_rhs4(const double fxl[4], const double fxr[4], const double fyb[4],
+                                          const double fyt[4], double dx, double dy, double rdx, double rdy,
+                                          double rhs[4]) {
+#if WENOFD_RECIP_DIV
+    (void)dx;
+    (void)dy;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) rhs[r] = -(fxr[r] - fxl[r]) * rdx + -(fyt[r] - fyb[r]) * rdy;
+#elif WENOFD_SHARED_DIV
+    (void)rdx;
+    (void)rdy;
+    bool ok = den_ok(dx) && den_ok(dy);
+    const double ex = rcp_refined(dx), ey = rcp_refined(dy);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        rhs[r] = div_exact(-(fxr[r] - fxl[r]), dx, ex, ok) + div_exact(-(fyt[r] - fyb[r]), dy, ey, ok);
+    if (__any(!ok)) {
+        if (!ok) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) rhs[r] = cell_rhs(fxl[r], fxr[r], fyb[r], fyt[r], dx, dy);
+        }
+    }
+#else
+    (void)rdx;
+    (void)rdy;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) rhs[r] = cell_rhs(fxl[r], fxr[r], fyb[r], fyt[r], dx, dy);
+#endif
 }
 
 // RK stage combination (src/WENOFD.cpp:394): (a Un + b U) + (c dt) rhs.
