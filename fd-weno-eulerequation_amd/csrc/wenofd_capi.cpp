@@ -252,7 +252,7 @@ int classify(wenofd_solver* h, float* logits_x = nullptr, float* logits_y = null
     if (h->sw_mode != WENOFD_SWITCH_MLP) return 0;
     if (!force && h->sw_since > 0 && h->sw_since < h->sw_every) return 0;
     h->sw_since = 0;
-    const int rad = h->sw_rad >= 0 ? h->sw_rad : h->sw_every;
+    const int rad = h->sw_rad >= 0 ? h->sw_rad : std::min(h->sw_every, 16);
     return timed(h, WENOFD_K_SWITCH, [&] {
         hipError_t e = launch_classify(0, h->U, h->g, h->sw_params, h->sw_thr, h->fl_raw_x, logits_x, h->stream);
         if (e == hipSuccess)
@@ -1153,7 +1153,7 @@ int wenofd_switch_set(wenofd_solver* h, int mode, const float* params, float thr
 
 int wenofd_switch_interval(wenofd_solver* h, int steps) {
     if (!h) return fail(WENOFD_E_ARG, "null handle");
-    if (steps < 1 || steps > 16) return fail(WENOFD_E_ARG, "the classification interval must be in [1, 16]");
+    if (steps < 1 || steps > 64) return fail(WENOFD_E_ARG, "the classification interval must be in [1, 64]");
     h->sw_every = steps;
     h->sw_since = 0;
     return 0;

@@ -227,8 +227,9 @@ int wenofd_switch_set(wenofd_solver* h, int mode, const float* params, float thr
  * [NY+1][NX]. */
 int wenofd_switch_classify(wenofd_solver* h, int dilate, unsigned char* flags_x, unsigned char* flags_y,
                            float* logits_x, float* logits_y);
-/* Classify every `steps` steps (1..16, default 1) and dilate the flags by
- * `steps` faces: a wave moves less than one cell per step at CFL <= 1. */
+/* Classify every `steps` steps (1..64, default 1) and, unless a radius is
+ * set (wenofd_switch_dilation), dilate the flags by min(`steps`, 16) faces:
+ * a wave moves less than one cell per step at CFL <= 1. */
 int wenofd_switch_interval(wenofd_solver* h, int steps);
 /* Dilation radius in faces (0..16) in place of the interval; -1 (the
  * default) follows the interval. A radius below the interval trades the
