@@ -281,7 +281,10 @@ def test_full_size_step_matches_oracle(tc, n, scheme, fp):
         o = O.Solver(tc, scheme, 2, 0.6, n, n, t_end=100.0, quad=quad)
         o.set_state(s.get_state())
         dt = s.step()
-        assert o.delta_t() == dt
+        if fp == 0:
+            assert o.delta_t() == dt
+        else:  # the contracted CFL step multiplies by 1/dx: a few ulps apart at most
+            assert abs(o.delta_t() - dt) <= 1e-13 * dt
         o.step(dt)
         U = s.get_state()
     if fp == 0:
