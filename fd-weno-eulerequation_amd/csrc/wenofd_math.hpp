@@ -202,8 +202,9 @@ __device__ __forceinline__ double div_exact(double n, double d, double r, bool& 
 // is required, which leaves every term that weighs more than 2^-60 of s normal.
 // FOLD (the build that divides by the reciprocal only): the linear weights
 // and the normalisation folded into the candidates, 3 instructions fewer per
-// reconstruction; the x sweeps use it (the y sweep, at its register cap,
-// would spill 20-36 dwords with it: 58 % slower).
+// reconstruction, and the reciprocal of the sum refined by one Newton step
+// (<= 11 ulp from 1/x; two steps are exact); the x sweeps use it (the y
+// sweep, at its register cap, would spill 20-36 dwords with it: 58 % slower).
 template <int S, bool FOLD = false>
 __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c, double d, double e,
                                                    double zp_lambda, bool& ok) {
@@ -268,7 +269,9 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
             const double W0 = E0 * (D1 * D2), W1 = E1 * (D0 * D2), W2 = E2 * (D0 * D1);
             const double s = (W0 + 6.0 * W1) + 3.0 * W2;
             ok = ok && s <= 0x1p1000 && (S != kZPI || s >= 0x1p-900);
-            const double r = rcp_refined(6.0 * s);
+            const double s6 = 6.0 * s;
+            const double r0 = __builtin_amdgcn_rcp(s6);
+            const double r = __builtin_fma(r0, __builtin_fma(-s6, r0, 1.0), r0);  // one Newton step: <= 11 ulp
             const double q0 = (2.0 * a - 7.0 * b) + 11.0 * c;    // 6 p0
             const double q1 = (-6.0 * b + 30.0 * c) + 12.0 * d;  // 36 p1
             const double q2 = (6.0 * c + 15.0 * d) - 3.0 * e;    // 18 p2
