@@ -200,11 +200,11 @@ __device__ __forceinline__ double div_exact(double n, double d, double r, bool& 
 // weights are first scaled by 1/M (M = bmax + eps, one reciprocal more) and
 // stay within [0, 6.3] per factor; where a beta ratio underflows, s >= 2^-900
 // is required, which leaves every term that weighs more than 2^-60 of s normal.
-// FOLD (the build that divides by the reciprocal only): the linear weights
-// and the normalisation folded into the candidates, 3 instructions fewer per
-// reconstruction, and the reciprocal of the sum refined by one Newton step
-// (<= 11 ulp from 1/x; two steps are exact); the x sweeps use it (the y
-// sweep, at its register cap, would spill 20-36 dwords with it: 58 % slower).
+// In the build that divides by the reciprocal the reciprocal of the weights'
+// sum takes one Newton step (<= 11 ulp from 1/x; two steps are exact), and
+// FOLD folds the linear weights and the normalisation into the candidates, 3
+// instructions fewer per reconstruction; the x sweeps use it (the y sweep, at
+// its register cap, would spill 20-36 dwords with it: 58 % slower).
 template <int S, bool FOLD = false>
 __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c, double d, double e,
                                                    double zp_lambda, bool& ok) {
@@ -283,7 +283,8 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
         w2 = 3.0 * (E2 * (D0 * D1));
         const double s = (w0 + w1) + w2;
         ok = ok && s <= 0x1p1000 && (S != kZPI || s >= 0x1p-900);
-        const double r = rcp_refined(s);
+        const double r0 = __builtin_amdgcn_rcp(s);
+        const double r = __builtin_fma(r0, __builtin_fma(-s, r0, 1.0), r0);  // one Newton step: <= 11 ulp
         w0 = w0 * r;
         w1 = w1 * r;
         w2 = w2 * r;
