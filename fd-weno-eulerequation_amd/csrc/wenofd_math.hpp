@@ -327,15 +327,41 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
 }
 
 // Device code of the build that divides by the reciprocal: divisions by the
-// density and the sound speed multiply by rcp_refined (v_rcp_f64 and two
-// Newton steps, the correctly rounded reciprocal on every sampled normal
-// operand) instead of the compiler's reciprocal, a full IEEE 1/x. A zero,
-// infinite or NaN density or sound speed (non-physical states) gives NaN
-// where the division gives inf or NaN.
+// density and the sound speed multiply by rcp_one (v_rcp_f64 and one Newton
+// step) instead of the compiler's reciprocal, a full IEEE 1/x, 1/c^2 is
+// (1/c)^2, and the sound speed's square root is sqrt_gs. A zero, infinite or
+// NaN density or sound speed (non-physical states) gives NaN where the
+// division gives inf or NaN.
 #if WENOFD_RECIP_DIV && defined(__HIP_DEVICE_COMPILE__)
 #define WENOFD_RCP_STATE 1
 #else
 #define WENOFD_RCP_STATE 0
+#endif
+
+#if WENOFD_RCP_STATE
+// The state's reciprocals in the build that divides by the reciprocal: one
+// Newton step after v_rcp_f64 (<= 11 ulp from 1/x on 4.2 M sampled normal
+// operands; v_rcp_f64 alone is 2^-24 accurate).
+__device__ __forceinline__ double rcp_one(double d) {
+    const double r0 = __builtin_amdgcn_rcp(d);
+    return __builtin_fma(r0, __builtin_fma(-d, r0, 1.0), r0);
+}
+
+// The sound speed's square root there: Goldschmidt's iteration from v_rsq_f64
+// (g = x y, h = y / 2, r = 1/2 - g h, g += g r, h += h r) and one Newton
+// correction g += (x - g^2) h: 8 instructions against the compiler's 17 (its
+// scaling of tiny operands and second correction), correctly rounded on 4.2 M
+// sampled operands in [2^-60, 2^61]; 0 for 0, NaN for a negative, infinite
+// or NaN operand (non-physical states).
+__device__ __forceinline__ double sqrt_gs(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(__builtin_fma(-g, g, x), h, g);
+    return x == 0.0 ? x : g;
+}
 #endif
 
 // Primitive variables with the reference's expression forms
@@ -352,14 +378,14 @@ WENOFD_HD Prim<T> primitive(T U0, T U1, T U2, T U3, double gamma) {
     Prim<T> q;
 #if WENOFD_RCP_STATE
     if constexpr (std::is_same<T, double>::value) {
-        const double r = rcp_refined(U0);
+        const double r = rcp_one(U0);
         q.rho = U0;
         q.irho = r;
         q.u = U1 * r;
         q.v = U2 * r;
         q.E = U3;
         q.p = (gamma - 1.0) * (q.E - (0.5 * q.rho) * (q.u * q.u + q.v * q.v));
-        q.c = sqrt((gamma * q.p) * r);
+        q.c = sqrt_gs((gamma * q.p) * r);
         return q;
     }
 #endif
@@ -449,8 +475,8 @@ WENOFD_HD EigenParts<T> eigen_parts(const T U[4], double gamma) {
     e.v = q.v;
     e.c = q.c;
 #if WENOFD_RCP_STATE
-    e.B1 = (gamma - 1.0) * rcp_refined(q.c * q.c);
-    e.ic = rcp_refined(q.c);
+    e.ic = rcp_one(q.c);
+    e.B1 = (gamma - 1.0) * (e.ic * e.ic);
     e.H = (q.E + q.p) * q.irho;
     e.nc = (D == 0 ? q.u : q.v) * e.ic;
 #else
