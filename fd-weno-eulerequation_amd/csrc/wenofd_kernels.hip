@@ -33,9 +33,10 @@ namespace wenofd {
 namespace WENOFD_FP_NS {
 
 // The build that divides by the reciprocal: its sweeps are capped at 3 waves
-// per SIMD where noted and recompute a face's IEEE fallback afresh (x sweeps,
-// and the WENO-ZPI y sweep: fewer registers there, more elsewhere and in the
-// build without contraction).
+// per SIMD where noted and recompute a face's IEEE fallback afresh (fewer
+// registers there; more in the build without contraction). Its y sweep then
+// fits the folded WENO normalisation with a 5-dword spill (without the
+// fresh fallback it spills 20-36: 58 % slower): −2.9 % y sweep.
 #if WENOFD_RECIP_DIV
 constexpr bool kCapRecip = true;
 #else
@@ -205,7 +206,7 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
                 Ur[r] = sU[r][(fr - base) % kRawRing][tx];
             }
             const int s0 = fr - 3 - base;  // ring position of point row fr-3
-            face_flux_dev<1, S, kCapRecip && S == kZPI>(
+            face_flux_dev<1, S, kCapRecip>(
                 Ul, Ur, [&](int s, int r, int z) { return sP[r][(s0 + s) % kRing][tx + z]; },
                 [&](int s, int r, int z) { return sM[r][(s0 + s) % kRing][tx + z]; }, g.gamma, g.zp_lambda, Fh);
         }

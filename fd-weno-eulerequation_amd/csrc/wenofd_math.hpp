@@ -203,8 +203,8 @@ __device__ __forceinline__ double div_exact(double n, double d, double r, bool& 
 // In the build that divides by the reciprocal the reciprocal of the weights'
 // sum takes one Newton step (<= 11 ulp from 1/x; two steps are exact), and
 // FOLD folds the linear weights and the normalisation into the candidates, 3
-// instructions fewer per reconstruction; the x sweeps use it (the y sweep, at
-// its register cap, would spill 20-36 dwords with it: 58 % slower).
+// instructions fewer per reconstruction (the sweeps' faces use it; the
+// hybrid's corrections and the probe keep the unfolded form).
 template <int S, bool FOLD = false>
 __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c, double d, double e,
                                                    double zp_lambda, bool& ok) {
@@ -668,9 +668,9 @@ __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double U
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const double hp =
-            weno5_shared_rcp<S, D == 0>(vp[0][k], vp[1][k], vp[2][k], vp[3][k], vp[4][k], zp_lambda, ok);
+            weno5_shared_rcp<S, true>(vp[0][k], vp[1][k], vp[2][k], vp[3][k], vp[4][k], zp_lambda, ok);
         const double hm =
-            weno5_shared_rcp<S, D == 0>(vm[4][k], vm[3][k], vm[2][k], vm[1][k], vm[0][k], zp_lambda, ok);
+            weno5_shared_rcp<S, true>(vm[4][k], vm[3][k], vm[2][k], vm[1][k], vm[0][k], zp_lambda, ok);
         vh[k] = hp + hm;
     }
     project_right<D>(R, vh, Fh);
