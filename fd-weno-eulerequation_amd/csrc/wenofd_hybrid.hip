@@ -78,8 +78,8 @@ __device__ __forceinline__ double dot4(const double a[4], const double f[4]) {
 template <int S>
 __device__ __forceinline__ double field_departure(const double vp[5], const double vm[5], double zp_lambda) {
     bool ok = true;
-    double w = weno5_shared_rcp<S>(vp[0], vp[1], vp[2], vp[3], vp[4], zp_lambda, ok) +
-               weno5_shared_rcp<S>(vm[4], vm[3], vm[2], vm[1], vm[0], zp_lambda, ok);
+    double w = weno5_shared_rcp<S, true>(vp[0], vp[1], vp[2], vp[3], vp[4], zp_lambda, ok) +
+               weno5_shared_rcp<S, true>(vm[4], vm[3], vm[2], vm[1], vm[0], zp_lambda, ok);
     if (__any(!ok)) {
         const double we = weno5<S>(vp[0], vp[1], vp[2], vp[3], vp[4], zp_lambda) +
                           weno5<S>(vm[4], vm[3], vm[2], vm[1], vm[0], zp_lambda);

@@ -203,8 +203,8 @@ __device__ __forceinline__ double div_exact(double n, double d, double r, bool& 
 // In the build that divides by the reciprocal the reciprocal of the weights'
 // sum takes one Newton step (<= 11 ulp from 1/x; two steps are exact), and
 // FOLD folds the linear weights and the normalisation into the candidates, 3
-// instructions fewer per reconstruction (the sweeps' faces use it; the
-// hybrid's corrections and the probe keep the unfolded form).
+// instructions fewer per reconstruction (the sweeps' faces and the hybrid's
+// corrections use it; the probe keeps the unfolded form).
 template <int S, bool FOLD = false>
 __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c, double d, double e,
                                                    double zp_lambda, bool& ok) {
