@@ -648,9 +648,31 @@ __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double U
     double ave[4], L[16], R[16];
 #pragma unroll
     for (int r = 0; r < 4; ++r) ave[r] = 0.5 * (Ul[r] + Ur[r]);
-    // (the mean state's divisions stay IEEE: through refined reciprocals they
-    // would cost the x sweep its third wave per SIMD)
+    // (the mean state's divisions stay IEEE in the build without contraction:
+    // through refined reciprocals they would cost the x sweep its third wave
+    // per SIMD)
+#if WENOFD_RECIP_DIV
+    // Rows 0, 2 and 3 of L share their B terms: with g = B1u f1 + B1v f2 - B1 f3,
+    // h = B2 f0 - g and k = (u_n/c) f0 - (1/c) f_n, they are (h + k)/2,
+    // f0 - h and (h - k)/2 (the reference's rows, reassociated: 10 instructions
+    // per vector instead of 13).
+    const EigenParts<double> e = eigen_parts<D>(ave, gamma);
+    eigen_matrices_from<D>(e, L, R);
+    const double B2h = 0.5 * e.B2, B1uh = 0.5 * e.B1u, B1vh = 0.5 * e.B1v, B1h = 0.5 * e.B1;
+    const double nch = 0.5 * e.nc, ich = 0.5 * e.ic;
+    auto project = [&](const double f[4], double v[4]) {
+        const double gh = __builtin_fma(-B1h, f[3], __builtin_fma(B1vh, f[2], B1uh * f[1]));
+        const double hh = __builtin_fma(B2h, f[0], -gh);
+        const double kh = __builtin_fma(nch, f[0], -(ich * f[D == 0 ? 1 : 2]));
+        v[0] = hh + kh;
+        v[1] = D == 0 ? __builtin_fma(e.v, f[0], -f[2]) : __builtin_fma(-e.u, f[0], f[1]);
+        v[2] = __builtin_fma(-2.0, hh, f[0]);
+        v[3] = hh - kh;
+    };
+#else
     eigen_matrices<D>(ave, gamma, L, R);
+    auto project = [&](const double f[4], double v[4]) { project_left<D>(L, f, v); };
+#endif
     bool ok = true;  // cleared where a reconstruction's division leaves its exact range
     double vp[5][4], vm[5][4];
 #pragma unroll
@@ -661,8 +683,8 @@ __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double U
             fp[r] = sp(s, r, 0);
             fm[r] = sm(s + 1, r, 0);
         }
-        project_left<D>(L, fp, vp[s]);
-        project_left<D>(L, fm, vm[s]);
+        project(fp, vp[s]);
+        project(fm, vm[s]);
     }
     double vh[4];
 #pragma unroll
