@@ -209,7 +209,25 @@ template <int S, bool FOLD = false>
 __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c, double d, double e,
                                                    double zp_lambda, bool& ok) {
     double b0, b1, b2, w0, w1, w2, p0, p1, p2;
-    weno_beta(a, b, c, d, e, b0, b1, b2);
+#if WENOFD_RECIP_DIV
+    constexpr bool kShared = FOLD;
+#else
+    constexpr bool kShared = false;  // the reference's expression forms
+#endif
+    if constexpr (kShared) {
+        // the smoothness indicators from shared differences (reassociated):
+        // with d1 = b - c and d2 = c - d, the middle stencil's second
+        // difference is d1 - d2 and the outer ones' first-order terms follow
+        // from their second differences
+        const double d1 = b - c, d2 = c - d;
+        const double s0 = (a - b) - d1, s1 = d1 - d2, s2 = d2 - (d - e);
+        const double u0 = __builtin_fma(-2.0, d1, s0), u1 = d1 + d2, u2 = __builtin_fma(2.0, d2, s2);
+        b0 = (kC13_12 * s0) * s0 + (0.25 * u0) * u0;
+        b1 = (kC13_12 * s1) * s1 + (0.25 * u1) * u1;
+        b2 = (kC13_12 * s2) * s2 + (0.25 * u2) * u2;
+    } else {
+        weno_beta(a, b, c, d, e, b0, b1, b2);
+    }
 #if WENOFD_RECIP_DIV
     {
         // per scheme: the factors D_k (squared, shifted betas) and E_k
@@ -695,7 +713,26 @@ __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double U
             weno5_shared_rcp<S, true>(vm[4][k], vm[3][k], vm[2][k], vm[1][k], vm[0][k], zp_lambda, ok);
         vh[k] = hp + hm;
     }
+#if WENOFD_RECIP_DIV
+    {
+        // R vh with R's shared terms (reassociated): F0 = (vh0 + vh3) + vh2,
+        // the normal momentum u_n F0 + c (vh3 - vh0), the tangential one
+        // u_t F0 -+ vh1, the energy H (vh0 + vh3) + c u_n (vh3 - vh0) + the
+        // shear and entropy columns
+        const double s03 = vh[0] + vh[3], dd = vh[3] - vh[0];
+        const double un = D == 0 ? e.u : e.v, ut = D == 0 ? e.v : e.u;
+        const double F0 = s03 + vh[2];
+        const double Fn = __builtin_fma(un, F0, e.c * dd);
+        const double Ft = __builtin_fma(ut, F0, D == 0 ? -vh[1] : vh[1]);
+        Fh[0] = F0;
+        Fh[1] = D == 0 ? Fn : Ft;
+        Fh[2] = D == 0 ? Ft : Fn;
+        Fh[3] = __builtin_fma(e.H, s03,
+                              __builtin_fma(e.c * un, dd, __builtin_fma(D == 0 ? -e.v : e.u, vh[1], (0.5 * e.uu) * vh[2])));
+    }
+#else
     project_right<D>(R, vh, Fh);
+#endif
     if (__any(!ok)) {
         // with FRESH, recompute from the mean state and fresh LDS reads: the
         // empty asm hides that they equal the fast path's, so none of its
