@@ -356,7 +356,7 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
 #define WENOFD_RCP_STATE 0
 #endif
 
-#if WENOFD_RCP_STATE
+#if WENOFD_RECIP_DIV
 // The state's reciprocals in the build that divides by the reciprocal: one
 // Newton step after v_rcp_f64 (<= 11 ulp from 1/x on 4.2 M sampled normal
 // operands; v_rcp_f64 alone is 2^-24 accurate).
@@ -663,21 +663,34 @@ WENOFD_HD void face_flux(const T Ul[4], const T Ur[4], GetP sp, GetM sm, double 
 template <int D, int S, bool FRESH, class GetP, class GetM>
 __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double Ur[4], GetP sp, GetM sm, double gamma,
                                               double zp_lambda, double Fh[4]) {
-    double ave[4], L[16], R[16];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) ave[r] = 0.5 * (Ul[r] + Ur[r]);
+    double L[16], R[16];
     // (the mean state's divisions stay IEEE in the build without contraction:
     // through refined reciprocals they would cost the x sweep its third wave
     // per SIMD)
 #if WENOFD_RECIP_DIV
+    // The eigen parts (velocities, sound speed, enthalpy) are ratios of the
+    // state's components, so the sum of the two points serves for their mean
+    // (the scaling by 2 is exact); the mean state itself is formed only for
+    // the IEEE fallback.
+    double sum[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sum[r] = Ul[r] + Ur[r];
+    const Prim<double> q = primitive(sum[0], sum[1], sum[2], sum[3], gamma);
+    struct {
+        double u, v, c, H;
+    } e{q.u, q.v, q.c, (q.E + q.p) * q.irho};
     // Rows 0, 2 and 3 of L share their B terms: with g = B1u f1 + B1v f2 - B1 f3,
     // h = B2 f0 - g and k = (u_n/c) f0 - (1/c) f_n, they are (h + k)/2,
     // f0 - h and (h - k)/2 (the reference's rows, reassociated: 10 instructions
-    // per vector instead of 13).
-    const EigenParts<double> e = eigen_parts<D>(ave, gamma);
-    eigen_matrices_from<D>(e, L, R);
-    const double B2h = 0.5 * e.B2, B1uh = 0.5 * e.B1u, B1vh = 0.5 * e.B1v, B1h = 0.5 * e.B1;
-    const double nch = 0.5 * e.nc, ich = 0.5 * e.ic;
+    // per vector instead of 13); B1 = (gamma - 1)/c^2, B2 = B1 (u^2 + v^2)/2,
+    // formed halved.
+    const double ich = 0.5 * rcp_one(q.c);
+    const double uuh = 0.5 * (q.u * q.u + q.v * q.v);
+    const double B1h = (2.0 * (gamma - 1.0)) * (ich * ich);
+    const double B2h = B1h * uuh, B1uh = B1h * q.u, B1vh = B1h * q.v;
+    const double nch = (D == 0 ? q.u : q.v) * ich;
+    (void)L;
+    (void)R;
     auto project = [&](const double f[4], double v[4]) {
         const double gh = __builtin_fma(-B1h, f[3], __builtin_fma(B1vh, f[2], B1uh * f[1]));
         const double hh = __builtin_fma(B2h, f[0], -gh);
@@ -688,6 +701,9 @@ __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double U
         v[3] = hh - kh;
     };
 #else
+    double ave[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ave[r] = 0.5 * (Ul[r] + Ur[r]);
     eigen_matrices<D>(ave, gamma, L, R);
     auto project = [&](const double f[4], double v[4]) { project_left<D>(L, f, v); };
 #endif
@@ -727,8 +743,7 @@ __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double U
         Fh[0] = F0;
         Fh[1] = D == 0 ? Fn : Ft;
         Fh[2] = D == 0 ? Ft : Fn;
-        Fh[3] = __builtin_fma(e.H, s03,
-                              __builtin_fma(e.c * un, dd, __builtin_fma(D == 0 ? -e.v : e.u, vh[1], (0.5 * e.uu) * vh[2])));
+        Fh[3] = __builtin_fma(e.H, s03, __builtin_fma(e.c * un, dd, __builtin_fma(D == 0 ? -e.v : e.u, vh[1], uuh * vh[2])));
     }
 #else
     project_right<D>(R, vh, Fh);
@@ -741,7 +756,11 @@ __device__ __forceinline__ void face_flux_dev(const double Ul[4], const double U
         double av[4], Fe[4];
         int k0 = 0;
 #pragma unroll
+#if WENOFD_RECIP_DIV
+        for (int r = 0; r < 4; ++r) av[r] = 0.5 * sum[r];
+#else
         for (int r = 0; r < 4; ++r) av[r] = ave[r];
+#endif
         if constexpr (FRESH) {
             asm volatile("" : "+v"(k0));
 #pragma unroll
