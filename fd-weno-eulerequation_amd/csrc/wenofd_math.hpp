@@ -232,6 +232,7 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
     {
         // per scheme: the factors D_k (squared, shifted betas) and E_k
         double D0, D1, D2, E0 = 1.0, E1 = 1.0, E2 = 1.0;
+        double tz2 = 0.0;  // WENO-Z: tau^2 (E_k = D_k + tau^2)
         if constexpr (S == kZPI) {
             // ZPI: d_k (1 + tau^2 / B_k^2 + 3.3 (1 - bmin / M) b_k / M) with
             // M = bmax + eps, scaled by rM^2 B_k^2 (rM = 1/M): every ratio lies
@@ -266,6 +267,7 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
             if constexpr (S == kZ) {
                 const double tau = fabs(b0 - b2);
                 const double t2 = tau * tau;
+                tz2 = t2;
                 E0 = D0 + t2;
                 E1 = D1 + t2;
                 E2 = D2 + t2;
@@ -284,7 +286,19 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
             // d_k scaled by 10: (1, 6, 3), folded with the normalisation into
             // the candidates (q_k = 6 p_k, integer coefficients): the
             // reconstruction is (W0 q0 + W1 6 q1 + W2 3 q2) / (6 (W0 + 6 W1 + 3 W2))
-            const double W0 = E0 * (D1 * D2), W1 = E1 * (D0 * D2), W2 = E2 * (D0 * D1);
+            const double D12 = D1 * D2, D02 = D0 * D2, D01 = D0 * D1;
+            double W0, W1, W2;
+            if constexpr (S == kZ) {
+                // (D_k + tau^2) D_i D_j = D0 D1 D2 + tau^2 D_i D_j: one FMA each
+                const double P = D0 * D12;
+                W0 = __builtin_fma(tz2, D12, P);
+                W1 = __builtin_fma(tz2, D02, P);
+                W2 = __builtin_fma(tz2, D01, P);
+            } else {
+                W0 = E0 * D12;
+                W1 = E1 * D02;
+                W2 = E2 * D01;
+            }
             const double s = (W0 + 6.0 * W1) + 3.0 * W2;
             ok = ok && s <= 0x1p1000 && (S != kZPI || s >= 0x1p-900);
             const double s6 = 6.0 * s;
