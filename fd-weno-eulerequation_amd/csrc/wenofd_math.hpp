@@ -214,18 +214,27 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
 #else
     constexpr bool kShared = false;  // the reference's expression forms
 #endif
+    // first differences of the stencil (the shared form below)
+    const double d0 = a - b, d1 = b - c, d2 = c - d, d3 = d - e;
+    // kShared: the smoothness indicators times 4, 13/3 s_k^2 + u_k^2 (three
+    // instructions each), from shared differences: the second differences s_k
+    // and the first-order terms u_k (a - 4b + 3c, b - d, 3c - 4d + e) follow
+    // from d0 .. d3. Every scheme's weights are invariant when the betas and
+    // eps scale together (eps is scaled by the same exact power of 2).
+    constexpr double kBetaScale = kShared ? 4.0 : 1.0;
+    (void)kBetaScale;
     if constexpr (kShared) {
-        // the smoothness indicators from shared differences (reassociated):
-        // with d1 = b - c and d2 = c - d, the middle stencil's second
-        // difference is d1 - d2 and the outer ones' first-order terms follow
-        // from their second differences
-        const double d1 = b - c, d2 = c - d;
-        const double s0 = (a - b) - d1, s1 = d1 - d2, s2 = d2 - (d - e);
+        const double s0 = d0 - d1, s1 = d1 - d2, s2 = d2 - d3;
         const double u0 = __builtin_fma(-2.0, d1, s0), u1 = d1 + d2, u2 = __builtin_fma(2.0, d2, s2);
-        b0 = (kC13_12 * s0) * s0 + (0.25 * u0) * u0;
-        b1 = (kC13_12 * s1) * s1 + (0.25 * u1) * u1;
-        b2 = (kC13_12 * s2) * s2 + (0.25 * u2) * u2;
+        constexpr double k13_3 = 13.0 / 3.0;
+        b0 = __builtin_fma(k13_3 * s0, s0, u0 * u0);
+        b1 = __builtin_fma(k13_3 * s1, s1, u1 * u1);
+        b2 = __builtin_fma(k13_3 * s2, s2, u2 * u2);
     } else {
+        (void)d0;
+        (void)d1;
+        (void)d2;
+        (void)d3;
         weno_beta(a, b, c, d, e, b0, b1, b2);
     }
 #if WENOFD_RECIP_DIV
@@ -237,7 +246,7 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
             // ZPI: d_k (1 + tau^2 / B_k^2 + 3.3 (1 - bmin / M) b_k / M) with
             // M = bmax + eps, scaled by rM^2 B_k^2 (rM = 1/M): every ratio lies
             // in [0, 1], so the scaled terms stay near 1 for any beta scale
-            const double eps = 1.0e-40;
+            const double eps = 1.0e-40 * kBetaScale;
             const double tau = fabs(b0 - b2);
             const double bmx12 = (b1 < b2) ? b2 : b1;
             const double bmax = (b0 < bmx12) ? bmx12 : b0;
@@ -257,7 +266,7 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
             E1 = D1 * (1.0 + la * (b1 * rM)) + t2;
             E2 = D2 * (1.0 + la * (b2 * rM)) + t2;
         } else {
-            const double eps = S == kJS ? 1.0e-6 : S == kZ ? 1.0e-20 : 1.0e-40;
+            const double eps = (S == kJS ? 1.0e-6 : S == kZ ? 1.0e-20 : 1.0e-40) * kBetaScale;
             const double B0 = S == kJS ? eps + b0 : b0 + eps;
             const double B1 = S == kJS ? eps + b1 : b1 + eps;
             const double B2 = S == kJS ? eps + b2 : b2 + eps;
@@ -304,10 +313,14 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
             const double s6 = 6.0 * s;
             const double r0 = __builtin_amdgcn_rcp(s6);
             const double r = __builtin_fma(r0, __builtin_fma(-s6, r0, 1.0), r0);  // one Newton step: <= 11 ulp
-            const double q0 = (2.0 * a - 7.0 * b) + 11.0 * c;    // 6 p0
-            const double q1 = (-6.0 * b + 30.0 * c) + 12.0 * d;  // 36 p1
-            const double q2 = (6.0 * c + 15.0 * d) - 3.0 * e;    // 18 p2
-            return ((W0 * q0 + W1 * q1) + W2 * q2) * r;
+            // q_k = 6 p_k = m_k c + r_k with (m_k) = 6 (1, 6, 3), so the
+            // weighted sum of the m_k c is 6 s c: the reconstruction is
+            // c + (W0 r0 + W1 r1 + W2 r2) / (6 s), the r_k from the differences
+            const double m12 = -12.0 * d2;
+            const double q0 = __builtin_fma(2.0, d0, -5.0 * d1);  // 2a - 7b + 5c
+            const double q1 = __builtin_fma(-6.0, d1, m12);       // -6b - 6c + 12d
+            const double q2 = __builtin_fma(3.0, d3, m12);        // -12c + 15d - 3e
+            return __builtin_fma((W0 * q0 + W1 * q1) + W2 * q2, r, c);
         }
         // d_k scaled by 10: (1, 6, 3)
         w0 = E0 * (D1 * D2);
