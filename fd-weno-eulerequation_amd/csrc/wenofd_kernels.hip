@@ -416,11 +416,14 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
 }
 
 // The contracted WENO-ZPI x sweep would take 170 VGPRs and run at 2 waves per
-// SIMD; capped at 3 it fits without spilling (-10 % sweep time), as does every
-// sweep of the build that divides by the reciprocal. The other sweeps fit by
-// themselves (their code is left to the default heuristics).
+// SIMD; capped at 3 it fits without spilling (-10 % sweep time). The other
+// sweeps of the build without contraction fit by themselves (their code is
+// left to the default heuristics). The build that divides by the reciprocal
+// caps its x sweeps at 4 waves per SIMD: they spill 32-42 dwords there (at
+// 3 waves none) and still run faster (x sweep -7 %, step -1.5 % (ZPI) to
+// -3.5 % (JS) at 2048^2).
 template <int S, int XT>
-constexpr int x_min_waves() { return S == kZPI || kCapRecip ? 3 : 1; }
+constexpr int x_min_waves() { return kCapRecip ? 4 : S == kZPI ? 3 : 1; }
 
 // With st, k.cdt holds the stage coefficient c and the step is c * st->dt.
 template <int S, int XT>
