@@ -419,7 +419,7 @@ __device__ __forceinline__ void flux_x_rk_tile(const double* __restrict__ Uin, c
 // SIMD; capped at 3 it fits without spilling (-10 % sweep time). The other
 // sweeps of the build without contraction fit by themselves (their code is
 // left to the default heuristics). The build that divides by the reciprocal
-// caps its x sweeps at 4 waves per SIMD: they spill 32-42 dwords there (at
+// caps its x sweeps at 4 waves per SIMD: they spill 33-39 dwords there (at
 // 3 waves none) and still run faster (x sweep -7 %, step -1.5 % (ZPI) to
 // -3.5 % (JS) at 2048^2).
 template <int S, int XT>

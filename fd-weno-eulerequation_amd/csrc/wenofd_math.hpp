@@ -240,9 +240,18 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
         const double s0 = d0 - d1, s1 = d1 - d2, s2 = d2 - d3;
         const double u0 = __builtin_fma(-2.0, d1, s0), u1 = d1 + d2, u2 = __builtin_fma(2.0, d2, s2);
         constexpr double k13_3 = 13.0 / 3.0;
-        b0 = __builtin_fma(k13_3 * s0, s0, u0 * u0);
-        b1 = __builtin_fma(k13_3 * s1, s1, u1 * u1);
-        b2 = __builtin_fma(k13_3 * s2, s2, u2 * u2);
+        if constexpr (S == kZPI) {
+            b0 = __builtin_fma(k13_3 * s0, s0, u0 * u0);
+            b1 = __builtin_fma(k13_3 * s1, s1, u1 * u1);
+            b2 = __builtin_fma(k13_3 * s2, s2, u2 * u2);
+        } else {
+            // JS, Z and ZP use the betas only as B_k = beta_k + eps and through
+            // B0 - B2: eps is added inside the indicators' FMAs
+            constexpr double eps4 = (S == kJS ? 1.0e-6 : S == kZ ? 1.0e-20 : 1.0e-40) * kBetaScale;
+            b0 = __builtin_fma(k13_3 * s0, s0, __builtin_fma(u0, u0, eps4));
+            b1 = __builtin_fma(k13_3 * s1, s1, __builtin_fma(u1, u1, eps4));
+            b2 = __builtin_fma(k13_3 * s2, s2, __builtin_fma(u2, u2, eps4));
+        }
     } else {
         (void)d0;
         (void)d1;
@@ -280,9 +289,9 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
             E2 = D2 * (1.0 + la * (b2 * rM)) + t2;
         } else {
             const double eps = (S == kJS ? 1.0e-6 : S == kZ ? 1.0e-20 : 1.0e-40) * kBetaScale;
-            const double B0 = S == kJS ? eps + b0 : b0 + eps;
-            const double B1 = S == kJS ? eps + b1 : b1 + eps;
-            const double B2 = S == kJS ? eps + b2 : b2 + eps;
+            const double B0 = kShared ? b0 : S == kJS ? eps + b0 : b0 + eps;
+            const double B1 = kShared ? b1 : S == kJS ? eps + b1 : b1 + eps;
+            const double B2 = kShared ? b2 : S == kJS ? eps + b2 : b2 + eps;
             D0 = B0 * B0;
             D1 = B1 * B1;
             D2 = B2 * B2;
@@ -323,16 +332,15 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
             }
             const double s = (W0 + 6.0 * W1) + 3.0 * W2;
             ok = ok && s <= 0x1p1000 && (S != kZPI || s >= 0x1p-900);
-            const double s6 = 6.0 * s;
-            const double r0 = __builtin_amdgcn_rcp(s6);
-            const double r = __builtin_fma(r0, __builtin_fma(-s6, r0, 1.0), r0);  // one Newton step: <= 11 ulp
+            const double r0 = __builtin_amdgcn_rcp(s);
+            const double r = __builtin_fma(r0, __builtin_fma(-s, r0, 1.0), r0);  // one Newton step: <= 11 ulp
             // q_k = 6 p_k = m_k c + r_k with (m_k) = 6 (1, 6, 3), so the
             // weighted sum of the m_k c is 6 s c: the reconstruction is
-            // c + (W0 r0 + W1 r1 + W2 r2) / (6 s), the r_k from the differences
-            const double m12 = -12.0 * d2;
-            const double q0 = __builtin_fma(2.0, d0, -5.0 * d1);  // 2a - 7b + 5c
-            const double q1 = __builtin_fma(-6.0, d1, m12);       // -6b - 6c + 12d
-            const double q2 = __builtin_fma(3.0, d3, m12);        // -12c + 15d - 3e
+            // c + (W0 r0 + W1 r1 + W2 r2) / (6 s), the r_k / 6 from the differences
+            const double t = -2.0 * d2;
+            const double q0 = __builtin_fma(kC1_3, d0, -kC5_6 * d1);  // (2a - 7b + 5c) / 6
+            const double q1 = t - d1;                                 // -b - c + 2d
+            const double q2 = __builtin_fma(0.5, d3, t);              // -2c + 5d/2 - e/2
             return __builtin_fma((W0 * q0 + W1 * q1) + W2 * q2, r, c);
         }
         // d_k scaled by 10: (1, 6, 3)
@@ -481,6 +489,31 @@ __device__ __forceinline__ Prim<double> primitive_sd(double U0, double U1, doubl
 // utils/equations.hpp:105-134 (physical flux), :365-393 (split).
 template <int D, class T>
 WENOFD_HD void flux_split(const T U[4], double gamma, T Fm[4], T Fp[4]) {
+#if WENOFD_RCP_STATE
+    if constexpr (std::is_same<T, double>::value) {
+        // formed halved: p/2 from rho (u^2 + v^2) = U1 u + U2 v, the flux's
+        // halves Fh from the normal momentum, w = (|u_n| + c)/2, and
+        // F-+ = Fh -+ w U (one FMA each)
+        const double r = rcp_one(U[0]);
+        const double u = U[1] * r, v = U[2] * r;
+        const double ph = (0.5 * (gamma - 1.0)) * __builtin_fma(-0.5, __builtin_fma(U[1], u, U[2] * v), U[3]);
+        const double c = sqrt_gs(((2.0 * gamma) * ph) * r);
+        const double un = D == 0 ? u : v;
+        const double mh = 0.5 * U[1 + D];
+        double Fh[4];
+        Fh[0] = mh;
+        Fh[1] = D == 0 ? __builtin_fma(mh, u, ph) : mh * u;
+        Fh[2] = D == 0 ? mh * v : __builtin_fma(mh, v, ph);
+        Fh[3] = __builtin_fma(0.5, U[3], ph) * un;
+        const double w = 0.5 * (fabs(un) + c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            Fm[k] = __builtin_fma(-w, U[k], Fh[k]);
+            Fp[k] = __builtin_fma(w, U[k], Fh[k]);
+        }
+        return;
+    }
+#endif
 #if WENOFD_SHARED_DIV
     Prim<T> q;
     if constexpr (std::is_same<T, double>::value) {
