@@ -32,11 +32,11 @@
 namespace wenofd {
 namespace WENOFD_FP_NS {
 
-// The build that divides by the reciprocal: its sweeps are capped at 3 waves
-// per SIMD where noted and recompute a face's IEEE fallback afresh (fewer
-// registers there; more in the build without contraction). Its y sweep then
-// fits the folded WENO normalisation with a 5-dword spill (without the
-// fresh fallback it spills 20-36: 58 % slower): −2.9 % y sweep.
+// The build that divides by the reciprocal: its sweeps are capped where noted
+// and recompute a face's IEEE fallback afresh (fewer registers there; more in
+// the build without contraction). Its y sweep then fits the folded WENO
+// normalisation at 3 waves per SIMD (without the fresh fallback it spilled
+// 20-36 dwords and ran 58 % slower).
 #if WENOFD_RECIP_DIV
 constexpr bool kCapRecip = true;
 #else
@@ -220,11 +220,12 @@ __device__ __forceinline__ void flux_y_tile(const double* __restrict__ U, double
 }
 
 // The WENO-Z and WENO-ZP y sweeps want a few more registers than 3 waves
-// per SIMD allow (Z: 172 VGPRs); capped there (a 1- and a 3-dword spill)
-// they run faster than at 2 waves (ZP: 2 %); for WENO-ZPI the 5-dword
-// spill costs more than it gains. The build that divides by the reciprocal
-// caps every scheme: JS and Z would otherwise take 246 VGPRs, and its ZPI
-// sweep runs 4 % faster capped (a 9-dword spill) than at 2 waves.
+// per SIMD allow (Z: 172 VGPRs); capped there (a 2-dword spill each) they
+// run faster than at 2 waves (ZP: 2 %); for WENO-ZPI the spill costs more
+// than it gains. The build that divides by the reciprocal caps every scheme:
+// JS and Z would otherwise take 246 VGPRs (capped, an 8-dword spill), and
+// its ZP and ZPI sweeps run faster capped (34- and 38-dword spills) than at
+// 2 waves without a spill (step -1.4 % ZP, -0.7 % ZPI).
 template <int S>
 constexpr int y_min_waves() { return S == kZ || S == kZP || kCapRecip ? 3 : 1; }
 
