@@ -395,9 +395,10 @@ __device__ __forceinline__ double weno5_shared_rcp(double a, double b, double c,
 // Device code of the build that divides by the reciprocal: divisions by the
 // density and the sound speed multiply by rcp_one (v_rcp_f64 and one Newton
 // step) instead of the compiler's reciprocal, a full IEEE 1/x, 1/c^2 is
-// (1/c)^2, and the sound speed's square root is sqrt_gs. A zero, infinite or
-// NaN density or sound speed (non-physical states) gives NaN where the
-// division gives inf or NaN.
+// (1/c)^2, and the sound speed's square root is sqrt_gs (at the faces, c and
+// 1/(2c) come from sqrt_half_rsqrt_gs). A zero, infinite or NaN density or
+// sound speed (non-physical states) gives NaN where the division gives inf
+// or NaN.
 #if WENOFD_RECIP_DIV && defined(__HIP_DEVICE_COMPILE__)
 #define WENOFD_RCP_STATE 1
 #else
