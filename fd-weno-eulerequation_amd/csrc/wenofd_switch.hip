@@ -16,13 +16,15 @@ namespace wenofd {
 
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 // ---------------------------------------------------------------------------
 // MLP inference on the matrix cores, transposed so that one layer's output
 // registers are the next layer's B operand without any data exchange:
 //   H1^T (32 x 16 samples) = W1 (32 x 16 features, 10 used) . X^T
 //   H2^T (32 x 16)         = W2 (32 x 32) . relu(H1^T + b1)
-//   z (1 x 16)             = w3 . relu(H2^T + b2) + b3   (VALU + 2 shuffles)
+//   z (1 x 16)             = w3 . relu(H2^T + b2) + b3   (VALU FMAs + 2 shuffles)
 // v_mfma_f32_16x16x16_bf16 lane layouts (lane l, j = 0..3):
 //   A[m = l%16][k = 4(l/16)+j], B[k = 4(l/16)+j][n = l%16], D[m = 4(l/16)+j][n = l%16].
 // One call handles 16 samples: lane l supplies features 4(l/16)..+3 of sample
@@ -62,7 +64,11 @@ __device__ __forceinline__ float mlp_group(const MlpRegs& R, bf16x4 x) {
         const f32x4 b = {R.b1[m][0], R.b1[m][1], R.b1[m][2], R.b1[m][3]};
         const f32x4 c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w1[m], x, b, 0, 0, 0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) h[m][j] = (__bf16)relu_f(c[j]);
+        for (int j = 0; j < 4; j += 2) {
+            const bf16x2 p = __builtin_convertvector((f32x2){relu_f(c[j]), relu_f(c[j + 1])}, bf16x2);
+            h[m][j] = p[0];
+            h[m][j + 1] = p[1];
+        }
     }
     float z = 0.0f;
 #pragma unroll
@@ -71,7 +77,7 @@ __device__ __forceinline__ float mlp_group(const MlpRegs& R, bf16x4 x) {
         f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w2[m][0], h[0], b, 0, 0, 0);
         d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w2[m][1], h[1], d, 0, 0, 0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) z += R.w3[m][j] * relu_f(d[j]);
+        for (int j = 0; j < 4; ++j) z = __builtin_fmaf(R.w3[m][j], relu_f(d[j]), z);
     }
     z += __shfl_xor(z, 16, 64);
     z += __shfl_xor(z, 32, 64);
