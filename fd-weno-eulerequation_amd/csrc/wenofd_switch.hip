@@ -18,6 +18,7 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
 
 // ---------------------------------------------------------------------------
 // MLP inference on the matrix cores, transposed so that one layer's output
@@ -65,9 +66,15 @@ __device__ __forceinline__ float mlp_group(const MlpRegs& R, bf16x4 x) {
         const f32x4 c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w1[m], x, b, 0, 0, 0);
 #pragma unroll
         for (int j = 0; j < 4; j += 2) {
-            const bf16x2 p = __builtin_convertvector((f32x2){relu_f(c[j]), relu_f(c[j + 1])}, bf16x2);
-            h[m][j] = p[0];
-            h[m][j + 1] = p[1];
+            // ReLU after the rounding to bf16, on the pair's sign bits: a
+            // bf16 read as a 16-bit integer is negative exactly when its sign
+            // bit is set, so a packed integer max with 0 zeroes the negative
+            // values (and -0, and NaNs with the sign bit)
+            const bf16x2 p = __builtin_convertvector((f32x2){c[j], c[j + 1]}, bf16x2);
+            const i16x2 r = __builtin_elementwise_max(__builtin_bit_cast(i16x2, p), (i16x2){0, 0});
+            const bf16x2 q = __builtin_bit_cast(bf16x2, r);
+            h[m][j] = q[0];
+            h[m][j + 1] = q[1];
         }
     }
     float z = 0.0f;
