@@ -127,9 +127,10 @@ __device__ __forceinline__ double lane_from_right(double v) {
 constexpr int kPatchThreads = 256;
 constexpr int kPatchBlocks = 512;  // per direction
 
-template <int S, int D>
-__device__ __forceinline__ void patch_face(const double* __restrict__ Uin, const Geom& g, unsigned face, unsigned m,
-                                           double* __restrict__ out) {
+// The face's six stencil points f-3 .. f+2 (rows of Uin along D).
+template <int D>
+__device__ __forceinline__ void patch_points(const double* __restrict__ Uin, const Geom& g, unsigned face,
+                                             double P[6][4]) {
     size_t p0, step;  // index of stencil point f-3 and the stride between points
     if constexpr (D == 0) {
         const int row = (int)(face / (unsigned)(g.nx + 1)), f = (int)(face % (unsigned)(g.nx + 1)) + 4;
@@ -140,23 +141,26 @@ __device__ __forceinline__ void patch_face(const double* __restrict__ Uin, const
         p0 = (size_t)(fr - 3) * g.pitch + i;
         step = (size_t)g.pitch;
     }
-    double Fp[6][4], Fm[6][4], ave[4];
 #pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        double P[4];
-        const size_t o = p0 + (size_t)s * step;
+    for (int s = 0; s < 6; ++s)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) P[r] = Uin[r * g.plane + o];
-        if (s == 2) {
+        for (int r = 0; r < 4; ++r) P[s][r] = Uin[r * g.plane + p0 + (size_t)s * step];
+}
+
+// The points' split along D and the face's mean state.
+template <int D>
+__device__ __forceinline__ void patch_split(const double P[6][4], const Geom& g, double Fp[6][4], double Fm[6][4],
+                                            double ave[4]) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) ave[r] = P[r];
-        }
-        if (s == 3) {
+    for (int r = 0; r < 4; ++r) ave[r] = 0.5 * (P[2][r] + P[3][r]);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) ave[r] = 0.5 * (ave[r] + P[r]);
-        }
-        flux_split<D>(P, g.gamma, Fm[s], Fp[s]);
-    }
+    for (int s = 0; s < 6; ++s) flux_split<D>(P[s], g.gamma, Fm[s], Fp[s]);
+}
+
+// The summed corrections of the flagged fields m of one face.
+template <int S, int D>
+__device__ __forceinline__ void patch_fields(const double Fp[6][4], const double Fm[6][4], const double ave[4],
+                                             const Geom& g, unsigned m, double* __restrict__ out) {
     const EigenParts<double> e = eigen_parts<D>(ave, g.gamma);
     double corr[4];
     bool first = true;
@@ -182,28 +186,33 @@ __device__ __forceinline__ void patch_face(const double* __restrict__ Uin, const
     for (int r = 0; r < 4; ++r) out[r] = corr[r];
 }
 
-// List entries e0, e0 + stride, ... below n of one direction. The next
-// entry's face is loaded while the current one is computed, and its mask
-// right after (needed only once its points are in); entries alternate
-// between two register sets, reloaded in place, and every load reads a valid
-// entry (clamped), so no load in flight is waited for early.
+// List entries e0, e0 + stride, ... below n of one direction. A face's points
+// are split as soon as they are in; the next face's points are then loaded
+// into the same registers and arrive while this face's fields are computed
+// (the entry of the face after it and its mask one face earlier still). Every
+// load reads a valid entry and valid points (the entry index clamped), so no
+// load in flight is waited for early.
 template <int S, int D>
 __device__ __forceinline__ void patch_run(const double* __restrict__ Uin, const Geom& g,
                                           const unsigned* __restrict__ list, const unsigned char* __restrict__ masks,
                                           int n, double* __restrict__ corr, int e0, int stride) {
     if (e0 >= n) return;
     auto entry = [&](int e) { return list[e < n ? e : n - 1]; };
-    unsigned fa = entry(e0);
-    unsigned ma = masks[fa];
-    for (int e = e0; e < n; e += 2 * stride) {
-        const int eb = e + stride;
-        unsigned fb = entry(eb);
-        patch_face<S, D>(Uin, g, fa, ma & 15u, corr + (size_t)fa * 4);
-        unsigned mb = masks[fb];
-        if (eb >= n) break;
-        fa = entry(eb + stride);
-        patch_face<S, D>(Uin, g, fb, mb & 15u, corr + (size_t)fb * 4);
-        ma = masks[fa];
+    unsigned fc = entry(e0);
+    unsigned mc = masks[fc];
+    unsigned fn = entry(e0 + stride);
+    double P[6][4];
+    patch_points<D>(Uin, g, fc, P);
+    for (int e = e0; e < n; e += stride) {
+        double Fp[6][4], Fm[6][4], ave[4];
+        patch_split<D>(P, g, Fp, Fm, ave);
+        const unsigned mn = masks[fn];
+        const unsigned fnn = entry(e + 2 * stride);
+        patch_points<D>(Uin, g, fn, P);
+        patch_fields<S, D>(Fp, Fm, ave, g, mc & 15u, corr + (size_t)fc * 4);
+        fc = fn;
+        mc = mn;
+        fn = fnn;
     }
 }
 
